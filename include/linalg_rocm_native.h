@@ -1,0 +1,128 @@
+/*
+ * C ABI of the MI355X-native (gfx950) LU solve path.
+ *
+ * Plain pointers and sizes only. Every matrix is row-major and square with its order as
+ * its row stride (lda == n), exactly like the reference's lu_decomposition / linear_solver
+ * contract. All array arguments are DEVICE pointers except `info`, which is host memory.
+ * The caller owns every buffer, workspace included: no entry point allocates or frees
+ * memory. Pivots are 1-based int64 LAPACK-style sequential row interchanges.
+ *
+ * Return value: LRN_OK (0) on success; -i when argument i (1-based) is invalid; a positive
+ * LRN_E_* code for runtime failures. A numerically singular matrix is NOT an error: the
+ * factorisation completes and reports the first zero pivot through `info`.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   include/matrix_operation/lu_decomposition.h:38-53   lu_decomposition<T, B>::apply / workspace_size
+ *   include/matrix_operation/linear_solver.h:46-64      linear_solver<T, B>::apply / workspace_size
+ *   src/matrix_operation/gpu/hipblas/lu_decomposition_hip.cpp:152-204  (ROCM_BLAS specialisation)
+ *   src/matrix_operation/gpu/hipblas/linear_solver_hip.cpp:242-306     (ROCM_BLAS specialisation)
+ */
+#ifndef LINALG_ROCM_NATIVE_H
+#define LINALG_ROCM_NATIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define LRN_API __declspec(dllexport)
+#else
+#define LRN_API __attribute__((visibility("default")))
+#endif
+
+enum lrn_status {
+    LRN_OK = 0,
+    LRN_E_HIP = 1,          /* a HIP runtime call failed */
+    LRN_E_TIMEOUT = 2,      /* a bounded inter-workgroup wait expired (result invalid) */
+    LRN_E_UNSUPPORTED = 3,  /* size outside what this build supports */
+    LRN_E_INTERNAL = 4
+};
+
+/* linear_solver_type values, identical to the reference enum
+ * (include/matrix_operation/linear_solver.h:15-21). */
+enum lrn_solver_type {
+    LRN_LU_LINEAR_SOLVER = 1,
+    LRN_LU_UPFRONT_LINEAR_SOLVER = 2,
+    LRN_CHOLESKY_LINEAR_SOLVER = 3,
+    LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER = 4
+};
+
+/* Human-readable text of a status code (static storage). */
+LRN_API const char* lrn_status_string(int status);
+
+/* ---- lu_decomposition<T, ROCM_NATIVE> ------------------------------------------------
+ * Replaces lu_decomposition<T, B>::workspace_size (lu_decomposition.h:53) and
+ * lu_decomposition<T, B>::apply (lu_decomposition.h:38-45; ROCM_BLAS version
+ * lu_decomposition_hip.cpp:152-192). In-place packed L\U (unit L below the diagonal),
+ * device pivots, host info. SYNCHRONOUS on `stream` (NULL = the null stream): returns
+ * after the factorisation has finished and *info has been written, as the reference's
+ * GPU specialisation does. */
+LRN_API int lrn_lu_decomposition_workspace_size_f64(int64_t lda, size_t* bytes);
+LRN_API int lrn_lu_decomposition_workspace_size_f32(int64_t lda, size_t* bytes);
+LRN_API int lrn_lu_decomposition_f64(double* m, int64_t lda, int64_t* pivot, void* workspace,
+    size_t workspace_size, int* info, void* stream);
+LRN_API int lrn_lu_decomposition_f32(float* m, int64_t lda, int64_t* pivot, void* workspace,
+    size_t workspace_size, int* info, void* stream);
+
+/* ---- linear_solver<T, ROCM_NATIVE> ---------------------------------------------------
+ * Replaces linear_solver<T, B>::workspace_size (linear_solver.h:64) and ::apply
+ * (linear_solver.h:46-53; ROCM_BLAS version linear_solver_hip.cpp:242-292). One right-hand
+ * side `x` of lda values, overwritten with the solution. LU mode factors `m` in place
+ * (pivots to `pivot`) and solves only if the matrix is nonsingular, leaving x unchanged
+ * otherwise (linear_solver_native.cpp:160-171); LU_UPFRONT reuses the caller's factors
+ * and pivots. ASYNCHRONOUS on `stream`. Cholesky modes return LRN_E_UNSUPPORTED. */
+LRN_API int lrn_linear_solver_workspace_size_f64(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_linear_solver_workspace_size_f32(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double* x, int type,
+    void* workspace, size_t workspace_size, void* stream);
+LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* x, int type,
+    void* workspace, size_t workspace_size, void* stream);
+
+/* ---- multi-RHS extension (no reference counterpart: the reference solves one RHS;
+ * docs/backend_roadmap.md:285 lists multi-RHS as future work) ---------------------------
+ * Same semantics as linear_solver with B a row-major lda x nrhs block (row stride ldb >=
+ * nrhs). Each column is solved as the single-RHS solver would solve it. ASYNCHRONOUS. */
+LRN_API int lrn_linear_solver_multi_workspace_size_f64(int64_t lda, int64_t nrhs, int type, size_t* bytes);
+LRN_API int lrn_linear_solver_multi_workspace_size_f32(int64_t lda, int64_t nrhs, int type, size_t* bytes);
+LRN_API int lrn_linear_solver_multi_f64(double* m, int64_t* pivot, int64_t lda, double* b, int64_t nrhs,
+    int64_t ldb, int type, void* workspace, size_t workspace_size, void* stream);
+LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, float* b, int64_t nrhs,
+    int64_t ldb, int type, void* workspace, size_t workspace_size, void* stream);
+
+/* ---- asynchronous factorisation (no reference counterpart) ---------------------------
+ * lu_decomposition without the host synchronisation: info is written to DEVICE memory
+ * (`info_dev`, one int) instead. Used by the solver and by callers that pipeline work. */
+LRN_API int lrn_getrf_async_f64(double* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream);
+LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream);
+
+/* Device tuning knobs (outer block, panel width, panel workgroups) for the autotuner. A
+ * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
+LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);
+LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg);
+
+/* ---- kernel test hooks (no reference counterpart) -------------------------------------
+ * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32.
+ * gemm_sub: C[M x N] -= A[M x K] * B[K x N] (row-major, leading dims lda/ldb/ldc).
+ * trsm: rows k..k+w-1, columns [c0, c1) of the lda x lda matrix a <- L11^-1 a, with L11 the
+ *       unit-lower w x w block at (k, k).
+ * panel: factor the column block a[k:lda, k:k+w] with partial pivoting (pivot[k..k+w) set);
+ *        `workspace` must hold lrn_lu_decomposition_workspace_size bytes. Asynchronous. */
+LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void* b, int64_t ldb, void* c,
+    int64_t ldc, int64_t m, int64_t n, int64_t k, void* stream);
+LRN_API int lrn_debug_trsm(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t c0, int64_t c1, void* stream);
+LRN_API int lrn_debug_panel(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* workspace,
+    size_t workspace_size, void* stream);
+
+/* Version string of this library. */
+LRN_API const char* lrn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LINALG_ROCM_NATIVE_H */

@@ -1,0 +1,16 @@
+"""MI355X-native (gfx950) LU solve path: lu_decomposition / linear_solver for the
+ROCM_NATIVE backend, backed by hand-written HIP kernels behind a C ABI.
+
+The compute lives in ``liblinalg_rocm_native.so`` (built from ``csrc/``); this package only
+loads it and passes device pointers and streams. There is no CPU fallback.
+"""
+from .lu import (  # noqa: F401
+    LinalgError,
+    get_tuning,
+    linear_solver,
+    linear_solver_type,
+    lu_decomposition,
+    set_tuning,
+)
+
+__all__ = ["lu_decomposition", "linear_solver", "linear_solver_type", "LinalgError", "set_tuning", "get_tuning"]

@@ -1,0 +1,48 @@
+"""Build recipe of the gfx950 library (hipcc, in-tree output).
+
+    python -m linearalgebra_amd.build
+
+compiles ``csrc/lrn_capi.hip`` (kernels + host driver + C ABI) into
+``liblinalg_rocm_native.so`` next to this file, and the C++ backend specialisation
+(``csrc/rocm_native_backend.cpp``) into ``liblinalg_rocm_native_cxx.so`` linked against it.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "liblinalg_rocm_native.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -packed-fp32-ops: no packed FP32 instructions are generated (they are avoided on purpose).
+# -ffp-contract=off: fused multiply-adds only where the source asks for fma()/MFMA.
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+    "-fvisibility=hidden", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"build failed: {' '.join(cmd)}")
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, "lrn_capi.hip")]
+    out = _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB, *srcs])
+    if verbose and out.strip():
+        print(out)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

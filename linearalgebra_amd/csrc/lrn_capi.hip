@@ -1,0 +1,596 @@
+// Host driver and C ABI of the gfx950 LU solve path.
+//
+// Factorisation (lu_decomposition): right-looking, blocked twice.
+//   outer block NB (default 256) -> the trailing update A22 -= L21 * U12 is one large MFMA GEMM
+//   inner panel W (<= 64)        -> cooperative getf2 panel, then laswp / TRSM / small GEMMs
+//                                   that finish the NB-wide block column and its U12 rows
+// Solve (linear_solver): permutation from the pivots, blocked forward and backward
+// substitution in two persistent kernels.
+//
+// Reference behaviour kept (paths relative to the reference repository):
+//   argument checks              src/matrix_operation/cpu/native/lu_decomposition_native.cpp:30-35,
+//                                src/matrix_operation/cpu/native/linear_solver_native.cpp:24-28
+//   info = first exact zero pivot, factorisation continues   lu_decomposition_native.cpp:147-157,200-208
+//   solve skipped on a singular matrix in LU mode            linear_solver_native.cpp:160-171
+//   synchronous lu_decomposition, asynchronous solver         docs/gpu_backend_design.md:64-99
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/linalg_rocm_native.h"
+#include "lrn_getrf_kernels.hpp"
+#include "lrn_getrs_kernels.hpp"
+
+namespace lrn {
+namespace {
+
+constexpr int kMaxW = 64;
+constexpr int kMaxInner = 16;  // inner panels per outer block column (row-map slots)
+constexpr unsigned kSpinLimit = 1u << 20;
+
+std::atomic<int> g_outer_nb{0};
+std::atomic<int> g_panel_w{0};
+std::atomic<int> g_panel_max_wg{0};
+
+int tuned_outer_nb(long long n)
+{
+    const int v = g_outer_nb.load();
+    if (v > 0) return v;
+    return n >= 2048 ? 256 : 128;
+}
+int tuned_panel_w()
+{
+    const int v = g_panel_w.load();
+    return (v > 0) ? std::min(v, kMaxW) : 64;
+}
+int tuned_panel_max_wg()
+{
+    const int v = g_panel_max_wg.load();
+    return (v > 0) ? std::min(v, kPanelMaxWG) : 128;
+}
+
+// Single-pass carve of a caller buffer, shared by the size query and the real call, in the
+// manner of the reference's workspace_arena (include/common/workspace_arena.h:35-73) but
+// with 256-byte alignment for device memory.
+struct Arena {
+    unsigned char* base;
+    size_t cap;
+    size_t off = 0;
+    bool ok = true;
+    explicit Arena(void* b = nullptr, size_t c = 0) : base(static_cast<unsigned char*>(b)), cap(c) {}
+    template <typename T> T* next(size_t count)
+    {
+        const size_t a = (off + 255) / 256 * 256;
+        off = a + sizeof(T) * count;
+        if (base == nullptr) return nullptr;
+        if (off > cap) {
+            ok = false;
+            return nullptr;
+        }
+        return reinterpret_cast<T*>(base + a);
+    }
+};
+
+template <typename T> struct GetrfLayout {
+    PanelWs<T> ws;
+};
+
+template <typename T> GetrfLayout<T> carve_getrf(Arena& ar)
+{
+    GetrfLayout<T> L;
+    L.ws.ctr = ar.next<unsigned>(64);
+    L.ws.info = ar.next<int>(64);
+    L.ws.err = ar.next<int>(64);
+    L.ws.sval = ar.next<double>(2 * kPanelMaxWG);
+    L.ws.sidx = ar.next<long long>(2 * kPanelMaxWG);
+    L.ws.srow = ar.next<T>(static_cast<size_t>(2) * kPanelMaxWG * kMaxW);
+    L.ws.sdiag = ar.next<T>(2 * kMaxW);
+    L.ws.rowmap = ar.next<int>(static_cast<size_t>(kMaxInner) * (1 + 4 * kMaxW));
+    return L;
+}
+
+template <typename T> struct GetrsLayout {
+    int* perm;
+    int* perm_tmp;
+    unsigned* flags_f;
+    unsigned* flags_b;
+    int* err;
+    T* y;
+};
+
+long long solve_blocks(long long n) { return (n + kSolveTB - 1) / kSolveTB; }
+
+template <typename T> GetrsLayout<T> carve_getrs(Arena& ar, long long n, long long nrhs)
+{
+    GetrsLayout<T> L;
+    L.perm = ar.next<int>(static_cast<size_t>(n));
+    L.perm_tmp = ar.next<int>(n > kPermLdsMax ? static_cast<size_t>(n) : 1);
+    L.flags_f = ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
+    L.flags_b = ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
+    L.err = ar.next<int>(64);
+    L.y = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(std::min<long long>(nrhs, kSolveMaxCols)));
+    return L;
+}
+
+struct DeviceProps {
+    int cus = 0;
+    int max_lds = 0;
+};
+
+DeviceProps& device_props(int dev)
+{
+    static std::mutex mu;
+    static std::vector<DeviceProps> props;
+    std::lock_guard<std::mutex> g(mu);
+    if (dev >= static_cast<int>(props.size())) props.resize(dev + 1);
+    DeviceProps& p = props[dev];
+    if (p.cus == 0) {
+        if (hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || p.cus <= 0)
+            p.cus = 1;
+        if (hipDeviceGetAttribute(&p.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            p.max_lds = 65536;
+    }
+    return p;
+}
+
+#define LRN_HIP(expr)                                   \
+    do {                                                \
+        if ((expr) != hipSuccess) return LRN_E_HIP;     \
+    } while (0)
+
+int next_pow2(long long v)
+{
+    int r = 1;
+    while (r < v) r <<= 1;
+    return r;
+}
+
+// ------------------------------------------------------------------------------- launches
+
+template <typename T> size_t panel_lds_bytes(int R, int W) { return static_cast<size_t>(R) * (W + 1) * sizeof(T); }
+
+template <typename T, int W>
+int launch_panel(T* A, long long lda, long long k, long long m, int w, long long* piv, const PanelWs<T>& ws,
+    unsigned& base, hipStream_t s)
+{
+    const int dev_max_wg = tuned_panel_max_wg();
+    int R, P;
+    if (m <= 256) {
+        R = std::max(next_pow2(m), W < 32 ? 32 : W);
+        P = 1;
+    } else {
+        R = next_pow2((m + dev_max_wg - 1) / dev_max_wg);
+        R = std::max(R, std::max(64, W));
+        const int rmax = (W <= 32) ? 512 : 256;
+        if (R > rmax) R = rmax;
+        P = static_cast<int>((m + R - 1) / R);
+        if (P > kPanelMaxWG) return LRN_E_UNSUPPORTED;
+    }
+    const size_t lds = panel_lds_bytes<T>(R, W);
+    auto kern = getf2_panel_kernel<T, W>;
+    LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+        static_cast<int>(lds)));
+    if (P > 1) {
+        int dev = 0;
+        LRN_HIP(hipGetDevice(&dev));
+        int per_cu = 0;
+        LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPanelThreads, lds));
+        if (per_cu <= 0 || static_cast<long long>(per_cu) * device_props(dev).cus < P) return LRN_E_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv, ws);
+    LRN_HIP(hipGetLastError());
+    if (P > 1) base += static_cast<unsigned>(P) * static_cast<unsigned>(w);
+    return LRN_OK;
+}
+
+template <typename T>
+int launch_laswp(T* A, long long lda, const int* rowmap, int W2, long long c0a, long long c1a, long long c0b,
+    long long c1b, hipStream_t s)
+{
+    const long long total = (c1a - c0a) + (c1b - c0b);
+    if (total <= 0) return LRN_OK;
+    const long long blocks = (total + kSwapCols - 1) / kSwapCols;
+    const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
+    hipLaunchKernelGGL(laswp_rowmap_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmap,
+        W2, c0a, c1a, c0b, c1b);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T, int W>
+int launch_trsm(T* A, long long lda, long long k, int w, long long c0, long long c1, hipStream_t s)
+{
+    if (c1 <= c0) return LRN_OK;
+    const long long blocks = (c1 - c0 + 255) / 256;
+    hipLaunchKernelGGL((trsm_llnu_kernel<T, W>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, k, w, c0,
+        c1);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+// C[M x N] -= A[M x K] * B[K x N]
+template <typename T>
+int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M,
+    long long N, long long K, hipStream_t s)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
+    constexpr int BM = 128, BN = 128, BK = 16;
+    const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    if (tm * tn > 0x7fffffffll) return LRN_E_UNSUPPORTED;
+    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s, A, lda,
+        B, ldb, C, ldc, static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tn));
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
+{
+    if (count <= 0) return LRN_OK;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s, p, v, count);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+#define LRN_TRY(expr)                         \
+    do {                                      \
+        const int _st = (expr);               \
+        if (_st != LRN_OK) return _st;        \
+    } while (0)
+
+// The blocked factorisation. All launches go to stream s; nothing waits on the host.
+//
+// Outer step K (block column [K, K+B)):
+//   1. factor the block column inner panel by inner panel; each inner panel's interchanges
+//      are applied inside the block column at once, and its L updates the rest of the block
+//      column (TRSM + GEMM restricted to columns [K, K+B));
+//   2. apply all of the block column's interchanges to the columns outside it, in order;
+//   3. U12 = L11^-1 A12 by blocked forward substitution (TRSM per inner panel + GEMM);
+//   4. A22 -= L21 * U12 in one GEMM with inner dimension B.
+template <typename T, int W>
+int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s)
+{
+    const long long lda = n;
+    const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
+    const int rm_stride = 1 + 4 * W;
+    unsigned base = 0;
+    LRN_TRY(launch_fill(ws.ctr, 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.info), 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.err), 0u, 64, s));
+    for (long long K = 0; K < n; K += NB) {
+        const long long B = std::min(NB, n - K);
+        const long long KB = K + B;
+        int slot = 0;
+        for (long long k = K; k < KB; k += W, ++slot) {
+            const int w = static_cast<int>(std::min<long long>(W, KB - k));
+            PanelWs<T> pw = ws;
+            pw.rowmap = ws.rowmap + slot * rm_stride;
+            LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
+            LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
+            if (k + w < KB) {
+                LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
+                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
+                    A + (k + w) * lda + (k + w), lda, n - (k + w), KB - (k + w), w, s));
+            }
+        }
+        // interchanges of the whole block column, outside it
+        slot = 0;
+        for (long long k = K; k < KB; k += W, ++slot)
+            LRN_TRY(launch_laswp<T>(A, lda, ws.rowmap + slot * rm_stride, 2 * W, 0, K, KB, n, s));
+        if (KB < n) {
+            for (long long k = K; k < KB; k += W) {
+                const int w = static_cast<int>(std::min<long long>(W, KB - k));
+                LRN_TRY((launch_trsm<T, W>(A, lda, k, w, KB, n, s)));
+                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + KB, lda, A + (k + w) * lda + KB,
+                    lda, KB - (k + w), n - KB, w, s));
+            }
+            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
+                n - KB, B, s));
+        }
+    }
+    return LRN_OK;
+}
+
+template <typename T>
+int getrf_dispatch(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s)
+{
+    const int w = tuned_panel_w();
+    if (w <= 32) return getrf_blocked<T, 32>(A, n, piv, ws, s);
+    return getrf_blocked<T, 64>(A, n, piv, ws, s);
+}
+
+template <typename T> size_t getrf_ws_bytes(long long)
+{
+    Arena ar;
+    carve_getrf<T>(ar);
+    return ar.off;
+}
+
+template <typename T> size_t getrs_ws_bytes(long long n, long long nrhs)
+{
+    Arena ar;
+    carve_getrs<T>(ar, n, nrhs);
+    return ar.off;
+}
+
+template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int type)
+{
+    Arena ar;
+    if (type == LRN_LU_LINEAR_SOLVER) carve_getrf<T>(ar);
+    carve_getrs<T>(ar, n, nrhs);
+    return ar.off;
+}
+
+// Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
+// without touching B (LU mode after a singular factorisation).
+template <typename T>
+int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nrhs, long long ldb,
+    const GetrsLayout<T>& L, const int* skip_dev, hipStream_t s)
+{
+    const size_t perm_lds = (n <= kPermLdsMax) ? static_cast<size_t>(n) * sizeof(int) : 0;
+    LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(perm_from_pivots_kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(perm_lds)));
+    hipLaunchKernelGGL(perm_from_pivots_kernel, dim3(1), dim3(64), perm_lds, s, piv, n, L.perm, L.perm_tmp);
+    LRN_HIP(hipGetLastError());
+    const long long nblk = solve_blocks(n);
+    for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
+        const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
+        LRN_TRY(launch_fill(L.flags_f, 0u, nblk, s));
+        LRN_TRY(launch_fill(L.flags_b, 0u, nblk, s));
+        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
+        hipLaunchKernelGGL((trsv_blocked_kernel<T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
+            L.perm, B + c0, ldb, L.y, static_cast<long long>(nc), nc, L.flags_f, kSpinLimit, L.err, skip_dev);
+        LRN_HIP(hipGetLastError());
+        hipLaunchKernelGGL((trsv_blocked_kernel<T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
+            L.perm, L.y, static_cast<long long>(nc), B + c0, ldb, nc, L.flags_b, kSpinLimit, L.err, skip_dev);
+        LRN_HIP(hipGetLastError());
+    }
+    return LRN_OK;
+}
+
+constexpr long long kMaxOrder = 1ll << 17;
+
+template <typename T>
+int lu_decomposition_impl(T* m, int64_t lda, int64_t* pivot, void* workspace, size_t ws_size, int* info, void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -2;
+    if (pivot == nullptr) return -3;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < getrf_ws_bytes<T>(lda)) return -4;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const auto L = carve_getrf<T>(ar);
+    LRN_TRY(getrf_dispatch<T>(m, lda, reinterpret_cast<long long*>(pivot), L.ws, s));
+    int host[2] = {0, 0};
+    LRN_HIP(hipMemcpyAsync(&host[0], L.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipMemcpyAsync(&host[1], L.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipStreamSynchronize(s));
+    if (host[1] != 0) return LRN_E_TIMEOUT;
+    if (info != nullptr) *info = host[0];
+    return LRN_OK;
+}
+
+template <typename T>
+int getrf_async_impl(T* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace, size_t ws_size, void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -2;
+    if (pivot == nullptr) return -3;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < getrf_ws_bytes<T>(lda)) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const auto L = carve_getrf<T>(ar);
+    LRN_TRY(getrf_dispatch<T>(m, lda, reinterpret_cast<long long*>(pivot), L.ws, s));
+    if (info_dev != nullptr) LRN_HIP(hipMemcpyAsync(info_dev, L.ws.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LRN_OK;
+}
+
+template <typename T>
+int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t ldb, int type, void* workspace,
+    size_t ws_size, void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -3;
+    if (b == nullptr) return -4;
+    if (nrhs < 0) return -5;
+    if (ldb < std::max<int64_t>(1, nrhs)) return -6;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (type != LRN_LU_LINEAR_SOLVER && type != LRN_LU_UPFRONT_LINEAR_SOLVER) return -7;
+    if (pivot == nullptr) return -2;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < solver_ws_bytes<T>(lda, std::max<int64_t>(nrhs, 1), type)) return -8;
+    if (nrhs == 0) return LRN_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const int* skip = nullptr;
+    if (type == LRN_LU_LINEAR_SOLVER) {
+        const auto F = carve_getrf<T>(ar);
+        LRN_TRY(getrf_dispatch<T>(m, lda, reinterpret_cast<long long*>(pivot), F.ws, s));
+        skip = F.ws.info;
+    }
+    const auto G = carve_getrs<T>(ar, lda, nrhs);
+    return getrs_run<T>(m, lda, reinterpret_cast<const long long*>(pivot), b, nrhs, ldb, G, skip, s);
+}
+
+template <typename T>
+int debug_panel_impl(T* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* ws, size_t ws_size, void* stream)
+{
+    if (ws_size < getrf_ws_bytes<T>(lda)) return -8;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(ws, ws_size);
+    const auto L = carve_getrf<T>(ar);
+    LRN_TRY(launch_fill(L.ws.ctr, 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.ws.info), 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.ws.err), 0u, 64, s));
+    unsigned base = 0;
+    return launch_panel<T, kMaxW>(a, lda, k, lda - k, w, reinterpret_cast<long long*>(pivot), L.ws, base, s);
+}
+
+}  // namespace
+}  // namespace lrn
+
+using namespace lrn;
+
+extern "C" {
+
+LRN_API const char* lrn_status_string(int status)
+{
+    switch (status) {
+    case LRN_OK: return "ok";
+    case LRN_E_HIP: return "HIP runtime call failed";
+    case LRN_E_TIMEOUT: return "bounded inter-workgroup wait expired";
+    case LRN_E_UNSUPPORTED: return "unsupported size or mode";
+    case LRN_E_INTERNAL: return "internal error";
+    default: return status < 0 ? "invalid argument" : "unknown status";
+    }
+}
+
+LRN_API const char* lrn_version(void) { return "linalg_rocm_native 0.1.0 (gfx950)"; }
+
+LRN_API int lrn_lu_decomposition_workspace_size_f64(int64_t lda, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = getrf_ws_bytes<double>(lda);
+    return LRN_OK;
+}
+LRN_API int lrn_lu_decomposition_workspace_size_f32(int64_t lda, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = getrf_ws_bytes<float>(lda);
+    return LRN_OK;
+}
+LRN_API int lrn_lu_decomposition_f64(double* m, int64_t lda, int64_t* pivot, void* workspace, size_t workspace_size,
+    int* info, void* stream)
+{
+    return lu_decomposition_impl<double>(m, lda, pivot, workspace, workspace_size, info, stream);
+}
+LRN_API int lrn_lu_decomposition_f32(float* m, int64_t lda, int64_t* pivot, void* workspace, size_t workspace_size,
+    int* info, void* stream)
+{
+    return lu_decomposition_impl<float>(m, lda, pivot, workspace, workspace_size, info, stream);
+}
+
+LRN_API int lrn_getrf_async_f64(double* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return getrf_async_impl<double>(m, lda, pivot, info_dev, workspace, workspace_size, stream);
+}
+LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return getrf_async_impl<float>(m, lda, pivot, info_dev, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_linear_solver_workspace_size_f64(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -3;
+    *bytes = solver_ws_bytes<double>(lda, 1, type);
+    return LRN_OK;
+}
+LRN_API int lrn_linear_solver_workspace_size_f32(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -3;
+    *bytes = solver_ws_bytes<float>(lda, 1, type);
+    return LRN_OK;
+}
+LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double* x, int type, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return solver_impl<double>(m, pivot, lda, x, 1, 1, type, workspace, workspace_size, stream);
+}
+LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* x, int type, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return solver_impl<float>(m, pivot, lda, x, 1, 1, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_linear_solver_multi_workspace_size_f64(int64_t lda, int64_t nrhs, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (nrhs < 0) return -2;
+    if (bytes == nullptr) return -4;
+    *bytes = solver_ws_bytes<double>(lda, std::max<int64_t>(nrhs, 1), type);
+    return LRN_OK;
+}
+LRN_API int lrn_linear_solver_multi_workspace_size_f32(int64_t lda, int64_t nrhs, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (nrhs < 0) return -2;
+    if (bytes == nullptr) return -4;
+    *bytes = solver_ws_bytes<float>(lda, std::max<int64_t>(nrhs, 1), type);
+    return LRN_OK;
+}
+LRN_API int lrn_linear_solver_multi_f64(double* m, int64_t* pivot, int64_t lda, double* b, int64_t nrhs, int64_t ldb,
+    int type, void* workspace, size_t workspace_size, void* stream)
+{
+    return solver_impl<double>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, float* b, int64_t nrhs, int64_t ldb,
+    int type, void* workspace, size_t workspace_size, void* stream)
+{
+    return solver_impl<float>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void* b, int64_t ldb, void* c,
+    int64_t ldc, int64_t m, int64_t n, int64_t k, void* stream)
+{
+    if (a == nullptr || b == nullptr || c == nullptr) return -2;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == 0)
+        return launch_gemm_sub<double>(static_cast<const double*>(a), lda, static_cast<const double*>(b), ldb,
+            static_cast<double*>(c), ldc, m, n, k, s);
+    if (dtype == 1)
+        return launch_gemm_sub<float>(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
+            static_cast<float*>(c), ldc, m, n, k, s);
+    return -1;
+}
+
+LRN_API int lrn_debug_trsm(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t c0, int64_t c1, void* stream)
+{
+    if (a == nullptr) return -2;
+    if (w <= 0 || w > kMaxW) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == 0) return launch_trsm<double, kMaxW>(static_cast<double*>(a), lda, k, w, c0, c1, s);
+    if (dtype == 1) return launch_trsm<float, kMaxW>(static_cast<float*>(a), lda, k, w, c0, c1, s);
+    return -1;
+}
+
+LRN_API int lrn_debug_panel(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    if (a == nullptr || pivot == nullptr || workspace == nullptr) return -2;
+    if (w <= 0 || w > kMaxW || k < 0 || k + w > lda) return -5;
+    if (dtype == 0) return debug_panel_impl<double>(static_cast<double*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
+    if (dtype == 1) return debug_panel_impl<float>(static_cast<float*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
+    return -1;
+}
+
+LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg)
+{
+    if (outer_nb > 0 && (outer_nb % 64) != 0) return -1;
+    if (panel_w > 0 && panel_w != 32 && panel_w != 64) return -2;
+    if (panel_max_wg > kPanelMaxWG) return -3;
+    g_outer_nb.store(outer_nb > 0 ? outer_nb : 0);
+    g_panel_w.store(panel_w > 0 ? panel_w : 0);
+    g_panel_max_wg.store(panel_max_wg > 0 ? panel_max_wg : 0);
+    return LRN_OK;
+}
+LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg)
+{
+    if (outer_nb) *outer_nb = tuned_outer_nb(1 << 20);
+    if (panel_w) *panel_w = tuned_panel_w();
+    if (panel_max_wg) *panel_max_wg = tuned_panel_max_wg();
+    return LRN_OK;
+}
+
+}  // extern "C"

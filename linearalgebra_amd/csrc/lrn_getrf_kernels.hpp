@@ -1,0 +1,491 @@
+// Kernels of the right-looking blocked LU factorisation (row-major, partial pivoting).
+//
+//   getf2_panel      cooperative panel factorisation of an m x w column block: pivot search by
+//                    wave-shuffle argmax, row swaps and rank-1 updates on an LDS-resident tile,
+//                    one bounded inter-workgroup barrier per column
+//   laswp_rowmap     applies one panel's row interchanges to the columns outside the panel
+//   trsm_llnu        U12 = L11^-1 A12 for a unit-lower w x w diagonal block (w <= 64)
+//   gemm_sub         C -= A * B on the matrix cores (FP64 / FP32 MFMA 16x16x4)
+//
+// Algorithm followed: src/matrix_operation/cpu/native/lu_decomposition_native.cpp:158-232
+// (panel), :247-271 (deferred interchanges), :364-380 (triangular solve), :382-422 (trailing
+// update as delta = L21*U12 followed by A22 -= delta).
+#pragma once
+
+#include <float.h>
+
+#include "lrn_device.hpp"
+
+namespace lrn {
+
+constexpr int kPanelThreads = 256;
+constexpr int kPanelMaxWG = 256;  // upper bound on workgroups of one panel launch
+
+template <typename T> struct PanelWs {
+    unsigned* ctr;     // barrier counter (monotonic over one factorisation)
+    double* sval;      // [2][kPanelMaxWG] published local maxima
+    long long* sidx;   // [2][kPanelMaxWG] their row indices
+    T* srow;           // [2][kPanelMaxWG][W] the candidate rows
+    T* sdiag;          // [2][W] the current diagonal row
+    int* info;         // first exactly-zero pivot, 1-based; 0 = none
+    int* err;          // raised when a bounded wait times out
+    int* rowmap;       // [1 + 4W]: count, dst[2W], src[2W] of the panel's net row moves
+};
+
+template <typename T> struct TypeMin;
+template <> struct TypeMin<double> { static constexpr double v = DBL_MIN; };
+template <> struct TypeMin<float> { static constexpr float v = FLT_MIN; };
+
+// Net row movement of one panel's interchanges. The swaps (i, piv[i]-1), i = k..k+w-1, are
+// replayed on position labels by one wave: lane l tracks position k+l, and positions at or
+// beyond k+w are tracked as "extras" in the lanes in order of first use. Output: pairs
+// (dst, src) meaning "row dst of the result is row src of the input".
+__device__ inline void panel_rowmap(const long long* pivs, long long k, int w, int W2, int* rowmap)
+{
+    const int lane = threadIdx.x;  // called by wave 0 only
+    long long c = k + lane;        // content label at position k+lane
+    long long e = -1, ec = -1;     // extra position handled by this lane, and its content
+    int ne = 0;
+    for (int ii = 0; ii < w; ++ii) {
+        const long long p = pivs[ii];
+        const long long ci = __shfl(c, ii, kWave);
+        long long cp;
+        if (p < k + w) {
+            const int lp = static_cast<int>(p - k);
+            cp = __shfl(c, lp, kWave);
+            if (lane == lp) c = ci;
+        } else {
+            const unsigned long long mask = __ballot(lane < ne && e == p);
+            if (mask) {
+                const int l2 = __ffsll(static_cast<long long>(mask)) - 1;
+                cp = __shfl(ec, l2, kWave);
+                if (lane == l2) ec = ci;
+            } else {
+                cp = p;
+                if (lane == ne) {
+                    e = p;
+                    ec = ci;
+                }
+                ++ne;
+            }
+        }
+        if (lane == ii) c = cp;
+    }
+    const bool moved = lane < w && c != k + lane;
+    const unsigned long long m1 = __ballot(moved);
+    const unsigned long long m2 = __ballot(lane < ne);
+    const unsigned long long lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int n1 = __popcll(m1);
+    if (moved) {
+        const int q = __popcll(m1 & lower);
+        rowmap[1 + q] = static_cast<int>(k + lane);
+        rowmap[1 + W2 + q] = static_cast<int>(c);
+    }
+    if (lane < ne) {
+        const int q = n1 + __popcll(m2 & lower);
+        rowmap[1 + q] = static_cast<int>(e);
+        rowmap[1 + W2 + q] = static_cast<int>(ec);
+    }
+    if (lane == 0) rowmap[0] = n1 + __popcll(m2);
+}
+
+// Cooperative panel factorisation of A[k:n, k:k+w] (row-major, stride lda).
+// Workgroup b owns rows k + b*R ... k + b*R + R - 1, holds them in LDS for the whole launch,
+// and all P workgroups agree on each column's pivot through one bounded barrier per column.
+// Requires R >= w (so workgroup 0 owns every diagonal row of the panel) and every one of the
+// P workgroups resident at once (the host sizes P from the occupancy of this kernel).
+template <typename T, int W>
+__global__ __launch_bounds__(kPanelThreads) void getf2_panel_kernel(T* __restrict__ A, long long lda,
+    long long k, long long m, int w, int R, int P, unsigned base, unsigned spin_limit,
+    long long* __restrict__ piv, PanelWs<T> ws)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int LD = W + 1;  // odd stride in 8-byte words: column reads hit distinct banks
+    T* tile = reinterpret_cast<T*>(lds_raw);
+    __shared__ T prow[W];
+    __shared__ T drow[W];
+    __shared__ double red_v[kPanelThreads / kWave];
+    __shared__ long long red_i[kPanelThreads / kWave];
+    __shared__ long long win_idx;
+    __shared__ long long pivs[W];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    const int b = blockIdx.x;
+    const long long row0 = k + static_cast<long long>(b) * R;  // first global row of this WG
+    const long long n = k + m;
+    const int rows_here = static_cast<int>(min(static_cast<long long>(R), n - row0));
+
+    // Load the workgroup's rows of the panel.
+    for (int idx = tid; idx < rows_here * w; idx += kPanelThreads) {
+        const int r = idx / w;
+        const int c = idx - r * w;
+        tile[r * LD + c] = A[(row0 + r) * lda + k + c];
+    }
+    __syncthreads();
+
+    // threads per row in the elimination step
+    const int tpr = (R >= kPanelThreads) ? 1 : (kPanelThreads / R);
+    const int rstep = kPanelThreads / tpr;
+
+    for (int j = 0; j < w; ++j) {
+        const long long i = k + j;  // global index of the diagonal row
+        const int par = j & 1;
+
+        // 1. local argmax over this workgroup's rows at or below the diagonal
+        double bv = -2.0;
+        long long bi = 0x7fffffffffffffffll;
+        for (int r = tid; r < rows_here; r += kPanelThreads) {
+            const long long g = row0 + r;
+            if (g >= i) {
+                const double v = pivot_key(tile[r * LD + j], g == i);
+                if (argmax_better(v, g, bv, bi)) {
+                    bv = v;
+                    bi = g;
+                }
+            }
+        }
+        wave_argmax(bv, bi);
+        if (lane == 0) {
+            red_v[wid] = bv;
+            red_i[wid] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int q = 1; q < kPanelThreads / kWave; ++q)
+                if (argmax_better(red_v[q], red_i[q], bv, bi)) {
+                    bv = red_v[q];
+                    bi = red_i[q];
+                }
+            red_v[0] = bv;
+            red_i[0] = bi;
+        }
+        __syncthreads();
+        bv = red_v[0];
+        bi = red_i[0];
+
+        long long p;  // global pivot row
+        if (P > 1) {
+            // 2. publish the local candidate (and, from workgroup 0, the diagonal row)
+            if (tid == 0) {
+                st_agent(&ws.sval[par * kPanelMaxWG + b], bv);
+                st_agent(&ws.sidx[par * kPanelMaxWG + b], bi);
+            }
+            if (tid < w && bv >= -1.0) {
+                const int rl = static_cast<int>(bi - row0);
+                st_agent(&ws.srow[(static_cast<size_t>(par) * kPanelMaxWG + b) * W + tid], tile[rl * LD + tid]);
+            }
+            if (b == 0 && tid < w) st_agent(&ws.sdiag[par * W + tid], tile[j * LD + tid]);
+            __threadfence();
+            __syncthreads();
+            // 3. one barrier per column
+            if (tid == 0) {
+                signal_counter(ws.ctr);
+                wait_counter(ws.ctr, base + static_cast<unsigned>(j + 1) * static_cast<unsigned>(P), spin_limit, ws.err);
+            }
+            __syncthreads();
+            // 4. global winner
+            double gv = -2.0;
+            long long gi = 0x7fffffffffffffffll;
+            if (tid < P) {
+                gv = ld_agent(&ws.sval[par * kPanelMaxWG + tid]);
+                gi = ld_agent(&ws.sidx[par * kPanelMaxWG + tid]);
+            }
+            wave_argmax(gv, gi);
+            if (lane == 0) {
+                red_v[wid] = gv;
+                red_i[wid] = gi;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                for (int q = 1; q < kPanelThreads / kWave; ++q)
+                    if (argmax_better(red_v[q], red_i[q], gv, gi)) {
+                        gv = red_v[q];
+                        gi = red_i[q];
+                    }
+                win_idx = gi;
+            }
+            __syncthreads();
+            p = win_idx;
+            const int wp = static_cast<int>((p - k) / R);
+            if (tid < w) {
+                prow[tid] = (wp == b) ? tile[static_cast<int>(p - row0) * LD + tid]
+                                      : ld_agent(&ws.srow[(static_cast<size_t>(par) * kPanelMaxWG + wp) * W + tid]);
+                drow[tid] = (b == 0) ? tile[j * LD + tid] : ld_agent(&ws.sdiag[par * W + tid]);
+            }
+        } else {
+            p = bi;
+            if (tid < w) {
+                prow[tid] = tile[static_cast<int>(p - row0) * LD + tid];
+                drow[tid] = tile[j * LD + tid];
+            }
+        }
+        __syncthreads();
+
+        // 5. interchange rows i and p inside the panel columns
+        if (p != i) {
+            if (b == 0 && tid < w) tile[j * LD + tid] = prow[tid];
+            const long long pl = p - row0;
+            if (pl >= 0 && pl < rows_here && tid < w) tile[static_cast<int>(pl) * LD + tid] = drow[tid];
+        }
+        if (b == 0 && tid == 0) {
+            piv[i] = p + 1;
+            pivs[j] = p;
+        }
+        __syncthreads();
+
+        // 6. multipliers and rank-1 update of the rows below the diagonal
+        const T d = prow[j];
+        if (d == T(0)) {
+            if (b == 0 && tid == 0 && *ws.info == 0) *ws.info = static_cast<int>(i + 1);
+        } else {
+            const bool use_recip = fabs(static_cast<double>(d)) >= static_cast<double>(TypeMin<T>::v);
+            const T rcp = use_recip ? T(1) / d : T(1);
+            for (int r = tid / tpr; r < rows_here; r += rstep) {
+                if (row0 + r <= i) continue;
+                T* rowp = tile + r * LD;
+                const T l = use_recip ? rowp[j] * rcp : rowp[j] / d;
+                for (int c = j + 1 + (tid % tpr); c < w; c += tpr) rowp[c] = rowp[c] - l * prow[c];
+                if ((tid % tpr) == 0) rowp[j] = l;
+            }
+        }
+        __syncthreads();
+    }
+
+    // Write the factored rows back.
+    for (int idx = tid; idx < rows_here * w; idx += kPanelThreads) {
+        const int r = idx / w;
+        const int c = idx - r * w;
+        A[(row0 + r) * lda + k + c] = tile[r * LD + c];
+    }
+    if (b == 0 && tid < kWave) panel_rowmap(pivs, k, w, 2 * W, ws.rowmap);
+}
+
+// Applies the net row moves of one panel (rowmap) to the columns [c0a, c1a) and [c0b, c1b).
+// Each workgroup moves a strip of kSwapCols columns: gather every source row into LDS, then
+// scatter to the destination rows, so a row that is both read and written is never raced.
+constexpr int kSwapCols = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_rowmap_kernel(T* __restrict__ A, long long lda,
+    const int* __restrict__ rowmap, int W2, long long c0a, long long c1a, long long c0b, long long c1b)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* buf = reinterpret_cast<T*>(lds_raw);  // [cnt][kSwapCols]
+    const int cnt = rowmap[0];
+    const int* dst = rowmap + 1;
+    const int* src = rowmap + 1 + W2;
+    const long long la = c1a - c0a;
+    const long long total = la + (c1b - c0b);
+    const long long s = static_cast<long long>(blockIdx.x) * kSwapCols;
+    for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
+        const int q = idx / kSwapCols;
+        const int cc = idx - q * kSwapCols;
+        const long long t = s + cc;
+        if (t < total) {
+            const long long col = (t < la) ? c0a + t : c0b + (t - la);
+            buf[idx] = A[static_cast<long long>(src[q]) * lda + col];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
+        const int q = idx / kSwapCols;
+        const int cc = idx - q * kSwapCols;
+        const long long t = s + cc;
+        if (t < total) {
+            const long long col = (t < la) ? c0a + t : c0b + (t - la);
+            A[static_cast<long long>(dst[q]) * lda + col] = buf[idx];
+        }
+    }
+}
+
+// U12 = L11^-1 * A12: rows k..k+w-1, columns [c0, c1). One thread per column; L11 (unit
+// lower, from the factored diagonal block) is broadcast from LDS. The rows are processed in
+// chunks of 16 held in registers; earlier chunks are re-read from memory (just written, so
+// cache hits). Each entry is reduced in increasing p with every product rounded before its
+// subtraction, the order of the reference's row-wise forward substitution.
+template <typename T, int W>
+__global__ __launch_bounds__(256) void trsm_llnu_kernel(T* __restrict__ A, long long lda, long long k, int w,
+    long long c0, long long c1)
+{
+    constexpr int CH = 16;
+    __shared__ T L[W][W + 1];
+    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+        const int r = idx / w;
+        const int c = idx - r * w;
+        L[r][c] = A[(k + r) * lda + k + c];
+    }
+    __syncthreads();
+    const long long col = c0 + static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= c1) return;
+    T* colp = A + k * lda + col;
+    for (int c = 0; c * CH < w; ++c) {
+        T x[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) x[q] = (c * CH + q < w) ? colp[(c * CH + q) * lda] : T(0);
+        for (int e = 0; e < c; ++e) {
+            T y[CH];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) y[q] = colp[(e * CH + q) * lda];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                T s = x[q];
+                const T* lrow = &L[(c * CH + q) < W ? (c * CH + q) : 0][e * CH];
+#pragma unroll
+                for (int p = 0; p < CH; ++p) s = s - lrow[p] * y[p];
+                x[q] = s;
+            }
+        }
+#pragma unroll
+        for (int q = 1; q < CH; ++q) {
+            T s = x[q];
+            const T* lrow = &L[(c * CH + q) < W ? (c * CH + q) : 0][c * CH];
+#pragma unroll
+            for (int p = 0; p < q; ++p) s = s - lrow[p] * x[p];
+            x[q] = s;
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (c * CH + q < w) colp[(c * CH + q) * lda] = x[q];
+    }
+}
+
+// ---------------------------------------------------------------------------------- GEMM
+
+template <typename T> struct Mfma16;
+// Operands of both: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k. Result element v of
+// lane l holds D[row(l, v)][l % 16]; the row order differs between the two instructions
+// (measured on gfx950 with one-hot operands).
+template <> struct Mfma16<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct Mfma16<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// C[M x N] -= A[M x K] * B[K x N], all row-major (leading dimensions lda, ldb, ldc).
+// 256 threads = 4 waves in a 2 x 2 arrangement; each wave owns a (BM/2) x (BN/2) block of
+// 16 x 16 MFMA tiles. A and B tiles are staged k-major in LDS so each lane's MFMA operand
+// (A[i][k] at lane i + 16k, B[k][j] at lane j + 16k) is one conflict-free LDS read.
+// The product is accumulated from zero and subtracted from C once at the end, the
+// reference's delta-then-subtract structure.
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, long long lda,
+    const T* __restrict__ B, long long ldb, T* __restrict__ C, long long ldc, int M, int N, int K,
+    int tiles_n)
+{
+    using MF = Mfma16<T>;
+    using acc_t = typename MF::acc_t;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int MI = WM / 16, NI = WN / 16;
+    constexpr int PAD = 2;
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int tile = blockIdx.x;
+    const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+    const long long row0 = static_cast<long long>(tm) * BM;
+    const long long col0 = static_cast<long long>(tn) * BN;
+
+    acc_t acc[MI][NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = acc_t{0, 0, 0, 0};
+
+    constexpr int A_PER = BM * BK / 256;
+    constexpr int B_PER = BK * BN / 256;
+    T ra[A_PER], rb[B_PER];
+
+    auto load_tiles = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < A_PER; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / BK, kk = idx - (idx / BK) * BK;
+            const long long gr = row0 + r;
+            const int gk = k0 + kk;
+            ra[q] = (gr < M && gk < K) ? A[gr * lda + gk] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; ++q) {
+            const int idx = tid + q * 256;
+            const int kk = idx / BN, c = idx - (idx / BN) * BN;
+            const long long gc = col0 + c;
+            const int gk = k0 + kk;
+            rb[q] = (gc < N && gk < K) ? B[static_cast<long long>(gk) * ldb + gc] : T(0);
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int q = 0; q < A_PER; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / BK, kk = idx - (idx / BK) * BK;
+            As[kk][r] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; ++q) {
+            const int idx = tid + q * 256;
+            const int kk = idx / BN, c = idx - (idx / BN) * BN;
+            Bs[kk][c] = rb[q];
+        }
+    };
+
+    load_tiles(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+        store_tiles();
+        __syncthreads();
+        if (k0 + BK < K) load_tiles(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T a[MI], bb[NI];
+            const int kr = kk + (lane >> 4);
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) a[mi] = As[kr][wm * WM + mi * 16 + (lane & 15)];
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni) bb[ni] = Bs[kr][wn * WN + ni * 16 + (lane & 15)];
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = MF::mma(a[mi], bb[ni], acc[mi][ni]);
+        }
+    }
+
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long gr = row0 + wm * WM + mi * 16 + MF::row(lane, v);
+                const long long gc = col0 + wn * WN + ni * 16 + (lane & 15);
+                if (gr < M && gc < N) {
+                    T* cp = C + gr * ldc + gc;
+                    *cp = *cp - acc[mi][ni][v];
+                }
+            }
+}
+
+// Zero-fill of small control areas (a kernel rather than hipMemsetAsync, so the sequence
+// stays valid inside a captured graph).
+__global__ void fill_u32_kernel(unsigned* p, unsigned v, long long count)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+}  // namespace lrn

@@ -1,0 +1,257 @@
+"""ctypes front end of the C oracle plus the input generators and error metrics the tests use.
+
+TEST INFRASTRUCTURE ONLY: see the package docstring.
+
+Reference correspondence:
+  getrf   -> lu_decomposition<T, NATIVE>::apply  (src/matrix_operation/cpu/native/lu_decomposition_native.cpp:447-459)
+  getrs   -> lu_solve, one column at a time       (src/matrix_operation/cpu/native/linear_solver_native.cpp:90-113)
+  gesv    -> linear_solver<T, NATIVE>::apply, LU  (linear_solver_native.cpp:160-171)
+  workspace_size -> lu_workspace_size<T>          (lu_decomposition_native.cpp:293-326)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblu_oracle.so")
+_lib = None
+
+_i64p = ctypes.POINTER(ctypes.c_longlong)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        for sfx, fp in (("d", _f64p), ("s", _f32p)):
+            f = getattr(L, f"ora_getrf_{sfx}")
+            f.argtypes = [fp, ctypes.c_longlong, _i64p]
+            f.restype = ctypes.c_int
+            f = getattr(L, f"ora_getrf_forced_{sfx}")
+            f.argtypes = [fp, ctypes.c_longlong, _i64p, _i64p, _f64p, _f64p]
+            f.restype = ctypes.c_int
+            f = getattr(L, f"ora_getrs_{sfx}")
+            f.argtypes = [fp, ctypes.c_longlong, _i64p, fp, ctypes.c_longlong, ctypes.c_longlong]
+            f.restype = ctypes.c_int
+            f = getattr(L, f"ora_gesv_{sfx}")
+            f.argtypes = [fp, ctypes.c_longlong, _i64p, fp, ctypes.c_longlong, ctypes.c_longlong]
+            f.restype = ctypes.c_int
+            f = getattr(L, f"ora_getrf_workspace_{sfx}")
+            f.argtypes = [ctypes.c_longlong]
+            f.restype = ctypes.c_size_t
+        L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def set_threads(t: int) -> None:
+    lib().ora_set_threads(int(t))
+
+
+def _sfx(dtype) -> tuple[str, object]:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "d", _f64p
+    if dt == np.float32:
+        return "s", _f32p
+    raise TypeError(f"oracle supports float32/float64, not {dt}")
+
+
+def _ptr(a: np.ndarray, p):
+    return a.ctypes.data_as(p)
+
+
+def getrf(a: np.ndarray, threads: int | None = None):
+    """Factor a copy of row-major square ``a``. Returns (lu, pivot[1-based int64], info)."""
+    a = np.ascontiguousarray(a)
+    n = a.shape[0]
+    assert a.shape == (n, n)
+    sfx, fp = _sfx(a.dtype)
+    if threads is not None:
+        set_threads(threads)
+    lu = a.copy()
+    piv = np.empty(n, dtype=np.int64)
+    info = getattr(lib(), f"ora_getrf_{sfx}")(_ptr(lu, fp), n, _ptr(piv, _i64p))
+    if info < 0:
+        raise MemoryError("oracle getrf: row buffer allocation failed")
+    return lu, piv, info
+
+
+def getrf_forced(a: np.ndarray, forced_pivot: np.ndarray):
+    """Replay the factorisation with every step's pivot row forced to ``forced_pivot``.
+
+    Returns (lu, pivot, info, colmax, chosen): colmax[i] is the largest |a| in column i at
+    step i (rows i..n-1), chosen[i] the magnitude of the forced pivot entry.
+    """
+    a = np.ascontiguousarray(a)
+    n = a.shape[0]
+    sfx, fp = _sfx(a.dtype)
+    lu = a.copy()
+    forced = np.ascontiguousarray(forced_pivot, dtype=np.int64)
+    piv = np.empty(n, dtype=np.int64)
+    colmax = np.zeros(n)
+    chosen = np.zeros(n)
+    info = getattr(lib(), f"ora_getrf_forced_{sfx}")(
+        _ptr(lu, fp), n, _ptr(forced, _i64p), _ptr(piv, _i64p), _ptr(colmax, _f64p), _ptr(chosen, _f64p))
+    if info < 0:
+        raise MemoryError("oracle getrf_forced: row buffer allocation failed")
+    return lu, piv, info, colmax, chosen
+
+
+def getrs(lu: np.ndarray, piv: np.ndarray, b: np.ndarray, threads: int | None = None):
+    """Solve with existing factors. ``b`` is a vector (n,) or row-major (n, nrhs)."""
+    lu = np.ascontiguousarray(lu)
+    n = lu.shape[0]
+    sfx, fp = _sfx(lu.dtype)
+    if threads is not None:
+        set_threads(threads)
+    x = np.array(b, dtype=lu.dtype, copy=True, order="C")
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    piv = np.ascontiguousarray(piv, dtype=np.int64)
+    getattr(lib(), f"ora_getrs_{sfx}")(_ptr(lu, fp), n, _ptr(piv, _i64p), _ptr(x, fp), nrhs, nrhs)
+    return x
+
+
+def gesv(a: np.ndarray, b: np.ndarray, threads: int | None = None):
+    """linear_solver LU mode: returns (lu, piv, x, info); x == b unchanged when singular."""
+    a = np.ascontiguousarray(a)
+    n = a.shape[0]
+    sfx, fp = _sfx(a.dtype)
+    if threads is not None:
+        set_threads(threads)
+    lu = a.copy()
+    piv = np.empty(n, dtype=np.int64)
+    x = np.array(b, dtype=a.dtype, copy=True, order="C")
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    info = getattr(lib(), f"ora_gesv_{sfx}")(_ptr(lu, fp), n, _ptr(piv, _i64p), _ptr(x, fp), nrhs, nrhs)
+    return lu, piv, x, info
+
+
+def reference_workspace_size(n: int, dtype=np.float64) -> int:
+    sfx, _ = _sfx(dtype)
+    return int(getattr(lib(), f"ora_getrf_workspace_{sfx}")(n))
+
+
+# ----------------------------------------------------------------------------- generators
+# Portable inputs: numpy's PCG64 bit generator with its standard-normal transform is a
+# documented, platform-independent stream for a given seed, so host, tests and fixtures
+# all see the same matrices.
+
+def gaussian(n: int, m: int | None = None, seed: int = 0, dtype=np.float64) -> np.ndarray:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (n, n) if m is None else (n, m)
+    return rng.standard_normal(shape).astype(dtype, copy=False)
+
+
+def diag_dominant(n: int, seed: int = 0, dtype=np.float64) -> np.ndarray:
+    """A + n*I with A iid N(0,1)."""
+    a = gaussian(n, seed=seed, dtype=np.float64)
+    a[np.diag_indices(n)] += n
+    return a.astype(dtype, copy=False)
+
+
+def haar_orthogonal(n: int, seed: int) -> np.ndarray:
+    """Q of the QR of a seeded Gaussian, with R's diagonal made positive (Mezzadri)."""
+    g = gaussian(n, seed=seed)
+    q, r = np.linalg.qr(g)
+    q *= np.sign(np.diag(r))[None, :]
+    return q
+
+
+def conditioned(n: int, kappa: float, seed: int = 0) -> np.ndarray:
+    """U diag(s) V^T with Haar U, V and s log-spaced from 1 to 1/kappa."""
+    u = haar_orthogonal(n, seed)
+    v = haar_orthogonal(n, seed + 1)
+    s = np.logspace(0.0, -np.log10(kappa), n)
+    return (u * s[None, :]) @ v.T
+
+
+# ----------------------------------------------------------------------------- pivots / metrics
+
+def pivots_to_perm(piv: np.ndarray) -> np.ndarray:
+    """Row permutation p with (P^T A)[i] = A[p[i]] for LAPACK-style sequential swaps."""
+    n = piv.shape[0]
+    perm = np.arange(n)
+    for i in range(n):
+        j = int(piv[i]) - 1
+        if j != i:
+            perm[i], perm[j] = perm[j], perm[i]
+    return perm
+
+
+def unpack_lu(lu: np.ndarray):
+    n = lu.shape[0]
+    lo = np.tril(lu, -1).astype(np.float64)
+    lo[np.diag_indices(n)] = 1.0
+    up = np.triu(lu).astype(np.float64)
+    return lo, up
+
+
+def reconstruction_error(a: np.ndarray, lu: np.ndarray, piv: np.ndarray) -> float:
+    """max|P^T A - L U| / (max|A| * n * eps) -- dimensionless, O(1) for a stable LU."""
+    lo, up = unpack_lu(lu)
+    perm = pivots_to_perm(piv)
+    n = a.shape[0]
+    eps = np.finfo(lu.dtype).eps
+    diff = np.abs(a.astype(np.float64)[perm] - lo @ up).max()
+    scale = max(np.abs(a).max(), np.finfo(np.float64).tiny)
+    return float(diff / (scale * n * eps))
+
+
+def backward_error(a: np.ndarray, x: np.ndarray, b: np.ndarray) -> float:
+    """Normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), per column max."""
+    a = a.astype(np.float64)
+    x2 = x.reshape(x.shape[0], -1).astype(np.float64)
+    b2 = b.reshape(b.shape[0], -1).astype(np.float64)
+    r = b2 - a @ x2
+    na = np.abs(a).sum(axis=1).max()
+    num = np.abs(r).max(axis=0)
+    den = na * np.abs(x2).max(axis=0) + np.abs(b2).max(axis=0)
+    return float((num / np.where(den > 0, den, 1.0)).max())
+
+
+def forward_error(x: np.ndarray, x_ref: np.ndarray) -> float:
+    """||x - x_ref||_inf / ||x_ref||_inf, per column max."""
+    x2 = x.reshape(x.shape[0], -1).astype(np.float64)
+    r2 = x_ref.reshape(x_ref.shape[0], -1).astype(np.float64)
+    num = np.abs(x2 - r2).max(axis=0)
+    den = np.abs(r2).max(axis=0)
+    return float((num / np.where(den > 0, den, 1.0)).max())
+
+
+def rel_diff(a: np.ndarray, b: np.ndarray) -> float:
+    """Normwise relative difference ||a - b||_F / ||b||_F."""
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def first_pivot_mismatch(p_gpu: np.ndarray, p_ref: np.ndarray) -> int:
+    """Index of the first differing pivot, or -1."""
+    d = np.nonzero(np.asarray(p_gpu) != np.asarray(p_ref))[0]
+    return int(d[0]) if d.size else -1
+
+
+__all__ = [
+    "build", "lib", "set_threads", "getrf", "getrf_forced", "getrs", "gesv",
+    "reference_workspace_size", "gaussian", "diag_dominant", "haar_orthogonal", "conditioned",
+    "pivots_to_perm", "unpack_lu", "reconstruction_error", "backward_error", "forward_error",
+    "rel_diff", "first_pivot_mismatch",
+]
