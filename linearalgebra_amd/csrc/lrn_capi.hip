@@ -20,11 +20,14 @@
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/linalg_rocm_native.h"
 #include "lrn_getrf_kernels.hpp"
 #include "lrn_getrs_kernels.hpp"
+#include "lrn_panel.hpp"
+#include "lrn_gemm_f64.hpp"
 
 namespace lrn {
 namespace {
@@ -46,7 +49,7 @@ int tuned_outer_nb(long long n)
 int tuned_panel_w()
 {
     const int v = g_panel_w.load();
-    return (v > 0) ? std::min(v, kMaxW) : 64;
+    return (v > 0) ? std::min(v, kMaxW) : 32;
 }
 int tuned_panel_max_wg()
 {
@@ -91,6 +94,8 @@ template <typename T> GetrfLayout<T> carve_getrf(Arena& ar)
     L.ws.srow = ar.next<T>(static_cast<size_t>(2) * kPanelMaxWG * kMaxW);
     L.ws.sdiag = ar.next<T>(2 * kMaxW);
     L.ws.rowmap = ar.next<int>(static_cast<size_t>(kMaxInner) * (1 + 4 * kMaxW));
+    L.ws.chunks = ar.next<Chunk>(static_cast<size_t>(2) * kPanel2MaxWG * 32);
+    L.ws.dchunks = ar.next<Chunk>(2 * 32);
     return L;
 }
 
@@ -143,6 +148,12 @@ DeviceProps& device_props(int dev)
         if ((expr) != hipSuccess) return LRN_E_HIP;     \
     } while (0)
 
+#define LRN_TRY(expr)                         \
+    do {                                      \
+        const int _st = (expr);               \
+        if (_st != LRN_OK) return _st;        \
+    } while (0)
+
 int next_pow2(long long v)
 {
     int r = 1;
@@ -154,10 +165,55 @@ int next_pow2(long long v)
 
 template <typename T> size_t panel_lds_bytes(int R, int W) { return static_cast<size_t>(R) * (W + 1) * sizeof(T); }
 
+int g_panel2_target_wg()
+{
+    const int v = g_panel_max_wg.load();
+    return (v > 0) ? std::min(v, kPanel2MaxWG) : 32;
+}
+
 template <typename T, int W>
 int launch_panel(T* A, long long lda, long long k, long long m, int w, long long* piv, const PanelWs<T>& ws,
     unsigned& base, hipStream_t s)
 {
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    const DeviceProps& props = device_props(dev);
+    if constexpr (W == 32) {
+        // one-round-trip cooperative panel
+        int R, P;
+        if (m <= 256) {
+            R = std::max(next_pow2(m), 32);
+            P = 1;
+        } else {
+            const int target = g_panel2_target_wg();
+            R = next_pow2((m + target - 1) / target);
+            R = std::max(R, 64);
+            if (R > 512) R = 512;
+            P = static_cast<int>((m + R - 1) / R);
+        }
+        if (P <= kPanel2MaxWG) {
+            const size_t lds = panel2_lds_bytes<T, W>(R, std::max(P, 1));
+            auto kern = getf2_panel2_kernel<T, W>;
+            LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                static_cast<int>(lds)));
+            if (P > 1) {
+                int per_cu = 0;
+                LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPanelThreads, lds));
+                if (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P) return LRN_E_UNSUPPORTED;
+            }
+            Panel2Ws<T> pw;
+            pw.slots = static_cast<Chunk*>(ws.chunks);
+            pw.dslots = static_cast<Chunk*>(ws.dchunks);
+            pw.info = ws.info;
+            pw.err = ws.err;
+            pw.rowmap = ws.rowmap;
+            hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv,
+                pw);
+            LRN_HIP(hipGetLastError());
+            base += static_cast<unsigned>(w);
+            return LRN_OK;
+        }
+    }
     const int dev_max_wg = tuned_panel_max_wg();
     int R, P;
     if (m <= 256) {
@@ -176,11 +232,9 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
         static_cast<int>(lds)));
     if (P > 1) {
-        int dev = 0;
-        LRN_HIP(hipGetDevice(&dev));
         int per_cu = 0;
         LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPanelThreads, lds));
-        if (per_cu <= 0 || static_cast<long long>(per_cu) * device_props(dev).cus < P) return LRN_E_UNSUPPORTED;
+        if (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P) return LRN_E_UNSUPPORTED;
     }
     hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv, ws);
     LRN_HIP(hipGetLastError());
@@ -219,13 +273,42 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
     long long N, long long K, hipStream_t s)
 {
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
-    constexpr int BM = 128, BN = 128, BK = 16;
-    const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-    if (tm * tn > 0x7fffffffll) return LRN_E_UNSUPPORTED;
-    hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s, A, lda,
-        B, ldb, C, ldc, static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tn));
-    LRN_HIP(hipGetLastError());
-    return LRN_OK;
+    if constexpr (std::is_same<T, double>::value) {
+        // interior tiles on the unchecked kernel, the ragged right / bottom strips on the checked one
+        const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0) &&
+                             (lda % 2 == 0) && (ldb % 2 == 0) && (K % kG64BK == 0);
+        const long long Mi = aligned ? M / kG64BM * kG64BM : 0;
+        const long long Ni = aligned ? N / kG64BN * kG64BN : 0;
+        auto launch = [&](bool check, const double* a, const double* b, double* c, long long m, long long nn) -> int {
+            if (m <= 0 || nn <= 0) return LRN_OK;
+            const long long tm = (m + kG64BM - 1) / kG64BM, tn = (nn + kG64BN - 1) / kG64BN;
+            if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
+            const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
+            const dim3 grid(static_cast<unsigned>(per_xcd * kXcds));
+            if (check)
+                hipLaunchKernelGGL(gemm_sub_f64_kernel<true>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
+                    static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
+                    static_cast<int>(tn), static_cast<int>(per_xcd));
+            else
+                hipLaunchKernelGGL(gemm_sub_f64_kernel<false>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
+                    static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
+                    static_cast<int>(tn), static_cast<int>(per_xcd));
+            LRN_HIP(hipGetLastError());
+            return LRN_OK;
+        };
+        LRN_TRY(launch(false, A, B, C, Mi, Ni));
+        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni));                       // right strip, all rows
+        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni));          // bottom strip
+        return LRN_OK;
+    } else {
+        constexpr int BM = 128, BN = 128, BK = 16;
+        const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+        if (tm * tn > 0x7fffffffll) return LRN_E_UNSUPPORTED;
+        hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s, A,
+            lda, B, ldb, C, ldc, static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tn));
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
 }
 
 int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
@@ -236,11 +319,18 @@ int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
     return LRN_OK;
 }
 
-#define LRN_TRY(expr)                         \
-    do {                                      \
-        const int _st = (expr);               \
-        if (_st != LRN_OK) return _st;        \
-    } while (0)
+
+// Clears the barrier counter, status words and every chunk tag before a factorisation, so
+// no tag left by an earlier call on the same workspace can match.
+template <typename T> int reset_panel_state(const PanelWs<T>& ws, hipStream_t s)
+{
+    LRN_TRY(launch_fill(ws.ctr, 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.info), 0u, 64, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.err), 0u, 64, s));
+    LRN_TRY(launch_fill(static_cast<unsigned*>(ws.chunks), 0u, 2ll * kPanel2MaxWG * 32 * 4, s));
+    LRN_TRY(launch_fill(static_cast<unsigned*>(ws.dchunks), 0u, 2ll * 32 * 4, s));
+    return LRN_OK;
+}
 
 // The blocked factorisation. All launches go to stream s; nothing waits on the host.
 //
@@ -258,9 +348,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
     const int rm_stride = 1 + 4 * W;
     unsigned base = 0;
-    LRN_TRY(launch_fill(ws.ctr, 0u, 64, s));
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.info), 0u, 64, s));
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(ws.err), 0u, 64, s));
+    LRN_TRY(reset_panel_state(ws, s));
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
@@ -425,10 +513,9 @@ int debug_panel_impl(T* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* 
     hipStream_t s = static_cast<hipStream_t>(stream);
     Arena ar(ws, ws_size);
     const auto L = carve_getrf<T>(ar);
-    LRN_TRY(launch_fill(L.ws.ctr, 0u, 64, s));
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.ws.info), 0u, 64, s));
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.ws.err), 0u, 64, s));
+    LRN_TRY(reset_panel_state(L.ws, s));
     unsigned base = 0;
+    if (w <= 32) return launch_panel<T, 32>(a, lda, k, lda - k, w, reinterpret_cast<long long*>(pivot), L.ws, base, s);
     return launch_panel<T, kMaxW>(a, lda, k, lda - k, w, reinterpret_cast<long long*>(pivot), L.ws, base, s);
 }
 

@@ -30,6 +30,8 @@ template <typename T> struct PanelWs {
     int* info;         // first exactly-zero pivot, 1-based; 0 = none
     int* err;          // raised when a bounded wait times out
     int* rowmap;       // [1 + 4W]: count, dst[2W], src[2W] of the panel's net row moves
+    void* chunks;      // chunk slots of the one-round-trip panel kernel (lrn_panel.hpp)
+    void* dchunks;
 };
 
 template <typename T> struct TypeMin;

@@ -67,7 +67,9 @@ def test_trsm_unit_lower(n, k, w, c0, c1, dev):
 
 
 @pytest.mark.parametrize("n,k,w", [(64, 0, 64), (127, 64, 63), (127, 0, 64), (300, 0, 64), (300, 100, 64),
-                                   (700, 5, 30), (2000, 0, 64), (5000, 128, 64), (20000, 0, 64)])
+                                   (700, 5, 30), (2000, 0, 64), (5000, 128, 64), (20000, 0, 64),
+                                   (32, 0, 32), (127, 96, 31), (300, 0, 32), (1000, 3, 32), (5000, 128, 32),
+                                   (20000, 0, 32), (33000, 64, 32)])
 def test_panel_vs_lapack(n, k, w, dev):
     import torch
     rng = np.random.default_rng(n + 7 * k + w)
