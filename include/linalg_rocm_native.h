@@ -104,6 +104,10 @@ LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info
  * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);
 LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg);
+/* Look-ahead (next block column factored on a helper stream beside the trailing update):
+ * on (1) or off (0, the default: with the current kernels the two streams do not overlap on
+ * gfx950 and the split adds launches). Process-wide. */
+LRN_API int lrn_set_lookahead(int enable);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32.

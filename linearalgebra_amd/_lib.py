@@ -37,6 +37,7 @@ SIGNATURES = {
     "lrn_debug_trsm": [_int, _vp, _i64, _i64, _int, _i64, _i64, _vp],
     "lrn_debug_panel": [_int, _vp, _i64, _i64, _int, _vp, _vp, _sz, _vp],
     "lrn_set_tuning": [_int, _int, _int],
+    "lrn_set_lookahead": [_int],
     "lrn_get_tuning": [ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)],
 }
 _RESTYPES = {"lrn_status_string": ctypes.c_char_p, "lrn_version": ctypes.c_char_p}

@@ -93,7 +93,9 @@ template <typename T> GetrfLayout<T> carve_getrf(Arena& ar)
     L.ws.sidx = ar.next<long long>(2 * kPanelMaxWG);
     L.ws.srow = ar.next<T>(static_cast<size_t>(2) * kPanelMaxWG * kMaxW);
     L.ws.sdiag = ar.next<T>(2 * kMaxW);
-    L.ws.rowmap = ar.next<int>(static_cast<size_t>(kMaxInner) * (1 + 4 * kMaxW));
+    // two sets of row maps: with the look-ahead the next block column is factored while the
+    // current one's maps are still being applied
+    L.ws.rowmap = ar.next<int>(static_cast<size_t>(2) * kMaxInner * (1 + 4 * kMaxW));
     L.ws.chunks = ar.next<Chunk>(static_cast<size_t>(2) * kPanel2MaxWG * 32);
     L.ws.dchunks = ar.next<Chunk>(2 * 32);
     return L;
@@ -163,6 +165,39 @@ int next_pow2(long long v)
 
 // ------------------------------------------------------------------------------- launches
 
+// One-time per (kernel, device) set-up: raise the dynamic-LDS limit to the device maximum
+// and cache the occupancy for a given LDS size, so launches make no extra runtime calls.
+int prepare_kernel(const void* f, size_t lds, int threads, int* per_cu)
+{
+    struct Entry {
+        const void* f;
+        size_t lds;
+        int threads;
+        int dev;
+        int per_cu;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    for (const Entry& e : cache)
+        if (e.f == f && e.lds >= lds && e.threads == threads && e.dev == dev && e.lds == lds) {
+            *per_cu = e.per_cu;
+            return LRN_OK;
+        }
+    hipFuncAttributes fa;
+    LRN_HIP(hipFuncGetAttributes(&fa, f));
+    const int max_dyn = device_props(dev).max_lds - static_cast<int>(fa.sharedSizeBytes);
+    if (static_cast<int>(lds) > max_dyn) return LRN_E_UNSUPPORTED;
+    if (lds > 0) LRN_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    int occ = 0;
+    LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, threads, lds));
+    cache.push_back(Entry{f, lds, threads, dev, occ});
+    *per_cu = occ;
+    return LRN_OK;
+}
+
 template <typename T> size_t panel_lds_bytes(int R, int W) { return static_cast<size_t>(R) * (W + 1) * sizeof(T); }
 
 int g_panel2_target_wg()
@@ -179,7 +214,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     LRN_HIP(hipGetDevice(&dev));
     const DeviceProps& props = device_props(dev);
     if constexpr (W == 32) {
-        // one-round-trip cooperative panel
+        // one-round-trip cooperative panel (lrn_panel.hpp)
         int R, P;
         if (m <= 256) {
             R = std::max(next_pow2(m), 32);
@@ -193,23 +228,29 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
         }
         if (P <= kPanel2MaxWG) {
             const size_t lds = panel2_lds_bytes<T, W>(R, std::max(P, 1));
-            auto kern = getf2_panel2_kernel<T, W>;
-            LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                static_cast<int>(lds)));
-            if (P > 1) {
-                int per_cu = 0;
-                LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPanelThreads, lds));
-                if (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P) return LRN_E_UNSUPPORTED;
-            }
             Panel2Ws<T> pw;
             pw.slots = static_cast<Chunk*>(ws.chunks);
             pw.dslots = static_cast<Chunk*>(ws.dchunks);
             pw.info = ws.info;
             pw.err = ws.err;
             pw.rowmap = ws.rowmap;
-            hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv,
-                pw);
-            LRN_HIP(hipGetLastError());
+            auto go = [&](auto kern) -> int {
+                int per_cu = 0;
+                LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
+                if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
+                hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit,
+                    piv, pw);
+                LRN_HIP(hipGetLastError());
+                return LRN_OK;
+            };
+            switch (panel_cpt(P)) {
+            case 1: LRN_TRY(go(getf2_panel3_kernel<T, W, 1>)); break;
+            case 2: LRN_TRY(go(getf2_panel3_kernel<T, W, 2>)); break;
+            case 3: LRN_TRY(go(getf2_panel3_kernel<T, W, 3>)); break;
+            case 4: LRN_TRY(go(getf2_panel3_kernel<T, W, 4>)); break;
+            case 5: LRN_TRY(go(getf2_panel3_kernel<T, W, 5>)); break;
+            default: LRN_TRY(go(getf2_panel3_kernel<T, W, 9>)); break;
+            }
             base += static_cast<unsigned>(w);
             return LRN_OK;
         }
@@ -229,12 +270,10 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     }
     const size_t lds = panel_lds_bytes<T>(R, W);
     auto kern = getf2_panel_kernel<T, W>;
-    LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-        static_cast<int>(lds)));
-    if (P > 1) {
+    {
         int per_cu = 0;
-        LRN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPanelThreads, lds));
-        if (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P) return LRN_E_UNSUPPORTED;
+        LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
+        if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
     }
     hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv, ws);
     LRN_HIP(hipGetLastError());
@@ -252,6 +291,22 @@ int launch_laswp(T* A, long long lda, const int* rowmap, int W2, long long c0a, 
     const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
     hipLaunchKernelGGL(laswp_rowmap_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmap,
         W2, c0a, c1a, c0b, c1b);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T>
+int launch_laswp_multi(T* A, long long lda, const int* rowmaps, int nmaps, int map_stride, int W2, long long c0a,
+    long long c1a, long long c0b, long long c1b, hipStream_t s)
+{
+    const long long total = (c1a - c0a) + (c1b - c0b);
+    if (total <= 0 || nmaps <= 0) return LRN_OK;
+    const long long blocks = (total + kSwapCols - 1) / kSwapCols;
+    const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
+    int per_cu = 0;
+    LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(laswp_multi_kernel<T>), lds, 256, &per_cu));
+    hipLaunchKernelGGL(laswp_multi_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmaps,
+        nmaps, map_stride, W2, c0a, c1a, c0b, c1b);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -284,7 +339,11 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             const long long tm = (m + kG64BM - 1) / kG64BM, tn = (nn + kG64BN - 1) / kG64BN;
             if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
             const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
-            const dim3 grid(static_cast<unsigned>(per_xcd * kXcds));
+            // persistent grid: two workgroups per compute unit at most
+            int dev = 0;
+            LRN_HIP(hipGetDevice(&dev));
+            const long long cap = 2ll * device_props(dev).cus;
+            const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<true>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
@@ -332,15 +391,105 @@ template <typename T> int reset_panel_state(const PanelWs<T>& ws, hipStream_t s)
     return LRN_OK;
 }
 
-// The blocked factorisation. All launches go to stream s; nothing waits on the host.
-//
-// Outer step K (block column [K, K+B)):
-//   1. factor the block column inner panel by inner panel; each inner panel's interchanges
-//      are applied inside the block column at once, and its L updates the rest of the block
-//      column (TRSM + GEMM restricted to columns [K, K+B));
-//   2. apply all of the block column's interchanges to the columns outside it, in order;
-//   3. U12 = L11^-1 A12 by blocked forward substitution (TRSM per inner panel + GEMM);
-//   4. A22 -= L21 * U12 in one GEMM with inner dimension B.
+// Per-device helper streams for the look-ahead: the block-column factorisation runs on a
+// stream restricted to the first `np` compute units while the rest of the trailing update
+// runs on a stream restricted to the others, so the panel's workgroups always find room.
+// Default priority, non-blocking.
+struct AsideStream {
+    int np = 0;
+    hipStream_t panel = nullptr;
+    hipStream_t gemm = nullptr;
+    hipEvent_t ready = nullptr;      // recorded on the caller's stream: next block column updated
+    hipEvent_t done = nullptr;       // recorded on the panel stream: next block column factored
+    hipEvent_t rest_done = nullptr;  // recorded on the GEMM stream: rest of the trailing update done
+};
+
+int aside_stream(int np, AsideStream** out)
+{
+    static std::mutex mu;
+    static std::vector<std::vector<AsideStream>> per_dev;
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    const int cus = device_props(dev).cus;
+    if (np >= cus) return LRN_E_UNSUPPORTED;
+    std::lock_guard<std::mutex> g(mu);
+    if (dev >= static_cast<int>(per_dev.size())) per_dev.resize(dev + 1);
+    for (auto& a : per_dev[dev])
+        if (a.np == np) {
+            *out = &a;
+            return LRN_OK;
+        }
+    AsideStream a;
+    a.np = np;
+    const int words = (cus + 31) / 32;
+    std::vector<uint32_t> mp(words, 0u), mg(words, 0u);
+    for (int c = 0; c < cus; ++c) (c < np ? mp : mg)[c / 32] |= 1u << (c % 32);
+    LRN_HIP(hipExtStreamCreateWithCUMask(&a.panel, static_cast<uint32_t>(words), mp.data()));
+    LRN_HIP(hipExtStreamCreateWithCUMask(&a.gemm, static_cast<uint32_t>(words), mg.data()));
+    LRN_HIP(hipEventCreateWithFlags(&a.ready, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a.rest_done, hipEventDisableTiming));
+    per_dev[dev].reserve(4);
+    if (per_dev[dev].size() >= 4) return LRN_E_INTERNAL;
+    per_dev[dev].push_back(a);
+    *out = &per_dev[dev].back();
+    return LRN_OK;
+}
+
+std::atomic<int> g_lookahead{0};
+
+// Factors block column [K, K+B) (rows K..n-1) inner panel by inner panel; each inner panel's
+// interchanges are applied inside the block column at once, and its L updates the rest of
+// the block column (TRSM + GEMM restricted to columns [K, K+B)).
+template <typename T, int W>
+int factor_block_column(T* A, long long n, long long K, long long B, long long* piv, const PanelWs<T>& ws,
+    int* rowmaps, unsigned& base, hipStream_t s)
+{
+    const long long lda = n;
+    const long long KB = K + B;
+    const int rm_stride = 1 + 4 * W;
+    int slot = 0;
+    for (long long k = K; k < KB; k += W, ++slot) {
+        const int w = static_cast<int>(std::min<long long>(W, KB - k));
+        PanelWs<T> pw = ws;
+        pw.rowmap = rowmaps + slot * rm_stride;
+        LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
+        LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
+        if (k + w < KB) {
+            LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
+            LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
+                A + (k + w) * lda + (k + w), lda, n - (k + w), KB - (k + w), w, s));
+        }
+    }
+    return LRN_OK;
+}
+
+// U12 rows of block column [K, K+B) for the columns [c0, c1): blocked forward substitution,
+// one TRSM per inner panel and a GEMM for the rows below it inside the block.
+template <typename T, int W>
+int u12_columns(T* A, long long n, long long K, long long B, long long c0, long long c1, hipStream_t s)
+{
+    const long long lda = n;
+    const long long KB = K + B;
+    if (c1 <= c0) return LRN_OK;
+    for (long long k = K; k < KB; k += W) {
+        const int w = static_cast<int>(std::min<long long>(W, KB - k));
+        LRN_TRY((launch_trsm<T, W>(A, lda, k, w, c0, c1, s)));
+        LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda,
+            KB - (k + w), c1 - c0, w, s));
+    }
+    return LRN_OK;
+}
+
+// The blocked factorisation, right-looking with a look-ahead of one block column.
+// For block column K (already factored; its row maps are in ws.rowmap):
+//   GEMM stream : interchanges on columns [0, K) and on the far columns [KB+B2, n), their U12
+//                 rows, and the far part of A22 -= L21 * U12;
+//   caller stream: interchanges, U12 rows and the A22 update of the next block column
+//                 [KB, KB+B2) only (the critical path);
+//   panel stream : factorisation of that next block column, as soon as it is updated.
+// The panel and GEMM streams are restricted to disjoint sets of compute units so the panel's
+// cooperative workgroups always find room. Nothing waits on the host.
 template <typename T, int W>
 int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s)
 {
@@ -348,37 +497,53 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
     const int rm_stride = 1 + 4 * W;
     unsigned base = 0;
+    AsideStream* side = nullptr;
+    const bool lookahead = g_lookahead.load() != 0 && n > NB;
+    if (lookahead) LRN_TRY(aside_stream(n > 16384 ? 64 : 32, &side));
+    const int set_stride = kMaxInner * (1 + 4 * kMaxW);
+    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
     LRN_TRY(reset_panel_state(ws, s));
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s)));
+    if (lookahead) LRN_HIP(hipEventRecord(side->done, s));
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
-        int slot = 0;
-        for (long long k = K; k < KB; k += W, ++slot) {
-            const int w = static_cast<int>(std::min<long long>(W, KB - k));
-            PanelWs<T> pw = ws;
-            pw.rowmap = ws.rowmap + slot * rm_stride;
-            LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
-            LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
-            if (k + w < KB) {
-                LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
-                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
-                    A + (k + w) * lda + (k + w), lda, n - (k + w), KB - (k + w), w, s));
-            }
-        }
-        // interchanges of the whole block column, outside it
-        slot = 0;
-        for (long long k = K; k < KB; k += W, ++slot)
-            LRN_TRY(launch_laswp<T>(A, lda, ws.rowmap + slot * rm_stride, 2 * W, 0, K, KB, n, s));
-        if (KB < n) {
-            for (long long k = K; k < KB; k += W) {
-                const int w = static_cast<int>(std::min<long long>(W, KB - k));
-                LRN_TRY((launch_trsm<T, W>(A, lda, k, w, KB, n, s)));
-                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + KB, lda, A + (k + w) * lda + KB,
-                    lda, KB - (k + w), n - KB, w, s));
-            }
+        const int nmaps = static_cast<int>((B + W - 1) / W);
+        int* const rms = maps(K);
+        if (!lookahead) {
+            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, KB, n, s));
+            if (KB >= n) break;
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
             LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
                 n - KB, B, s));
+            LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s)));
+            continue;
         }
+        const long long B2 = std::min(NB, n - KB);  // next block column (0 at the end)
+        const long long far0 = KB + B2;
+        // far columns and the left columns: GEMM stream, after block column K is factored
+        LRN_HIP(hipStreamWaitEvent(side->gemm, side->done, 0));
+        LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, far0, n, side->gemm));
+        if (far0 < n) {
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, far0, n, side->gemm)));
+            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
+                n - far0, B, side->gemm));
+        }
+        LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
+        // next block column: caller stream
+        LRN_HIP(hipStreamWaitEvent(s, side->done, 0));
+        if (B2 > 0) {
+            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, s));
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, s)));
+            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
+                B, s));
+            LRN_HIP(hipEventRecord(side->ready, s));
+            LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
+            LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel)));
+            LRN_HIP(hipEventRecord(side->done, side->panel));
+        }
+        // the next step's caller-stream work needs this step's far update (its columns)
+        LRN_HIP(hipStreamWaitEvent(s, side->rest_done, 0));
     }
     return LRN_OK;
 }
@@ -420,8 +585,10 @@ int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nr
     const GetrsLayout<T>& L, const int* skip_dev, hipStream_t s)
 {
     const size_t perm_lds = (n <= kPermLdsMax) ? static_cast<size_t>(n) * sizeof(int) : 0;
-    LRN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(perm_from_pivots_kernel),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(perm_lds)));
+    {
+        int per_cu = 0;
+        LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(perm_from_pivots_kernel), perm_lds, 64, &per_cu));
+    }
     hipLaunchKernelGGL(perm_from_pivots_kernel, dim3(1), dim3(64), perm_lds, s, piv, n, L.perm, L.perm_tmp);
     LRN_HIP(hipGetLastError());
     const long long nblk = solve_blocks(n);
@@ -660,6 +827,12 @@ LRN_API int lrn_debug_panel(int dtype, void* a, int64_t lda, int64_t k, int w, i
     if (dtype == 0) return debug_panel_impl<double>(static_cast<double*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
     if (dtype == 1) return debug_panel_impl<float>(static_cast<float*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
     return -1;
+}
+
+LRN_API int lrn_set_lookahead(int enable)
+{
+    g_lookahead.store(enable ? 1 : 0);
+    return LRN_OK;
 }
 
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg)

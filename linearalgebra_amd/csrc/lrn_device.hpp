@@ -96,3 +96,70 @@ __device__ __forceinline__ void signal_counter(unsigned* ctr)
 
 
 }  // namespace lrn
+
+namespace lrn {
+
+// Wave-wide maximum of a double (all 64 lanes active). Within each row of 16 lanes by DPP
+// (swap neighbours, swap pairs, half-row mirror, row mirror), then across the four rows by
+// reading lanes 0, 16, 32 and 48. Every lane returns the same value.
+__device__ __forceinline__ double dpp_max_step(double v, int ctrl_sel)
+{
+    const long long bits = __double_as_longlong(v);
+    int lo = static_cast<int>(bits), hi = static_cast<int>(bits >> 32);
+    int lo2, hi2;
+    switch (ctrl_sel) {
+    case 0:
+        lo2 = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+        hi2 = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+        break;
+    case 1:
+        lo2 = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
+        hi2 = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+        break;
+    case 2:
+        lo2 = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);
+        hi2 = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+        break;
+    default:
+        lo2 = __builtin_amdgcn_mov_dpp(lo, 0x140, 0xF, 0xF, false);
+        hi2 = __builtin_amdgcn_mov_dpp(hi, 0x140, 0xF, 0xF, false);
+        break;
+    }
+    const double o = __longlong_as_double((static_cast<long long>(hi2) << 32) | static_cast<unsigned>(lo2));
+    return o > v ? o : v;
+}
+
+__device__ __forceinline__ double read_lane_f64(double v, int lane)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+__device__ __forceinline__ double wave_max_f64(double v)
+{
+    v = dpp_max_step(v, 0);
+    v = dpp_max_step(v, 1);
+    v = dpp_max_step(v, 2);
+    v = dpp_max_step(v, 3);
+    const double a = read_lane_f64(v, 0), b = read_lane_f64(v, 16);
+    const double c = read_lane_f64(v, 32), d = read_lane_f64(v, 48);
+    const double ab = a > b ? a : b, cd = c > d ? c : d;
+    return ab > cd ? ab : cd;
+}
+
+// Wave argmax for the pivot search when lanes hold candidates in increasing row order:
+// the maximum key, and the row of the lowest lane holding it. Keys are never NaN.
+__device__ __forceinline__ void wave_argmax_ordered(double& key, long long& row)
+{
+    const double m = wave_max_f64(key);
+    const unsigned long long hits = __ballot(key == m);
+    const int l = __ffsll(static_cast<long long>(hits)) - 1;
+    const long long r = (static_cast<long long>(__builtin_amdgcn_readlane(static_cast<int>(row >> 32), l)) << 32) |
+                        static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(row), l));
+    key = m;
+    row = r;
+}
+
+}  // namespace lrn

@@ -114,8 +114,12 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
     __shared__ double As[2][BK][BM + PADA];
     __shared__ double Bs[2][BK][BN + PADB];
 
+    // persistent: workgroup g handles virtual tiles g, g + grid, ... (all workgroups see the
+    // same trip count, so the barriers inside the loop are uniform)
+    const int nvirt = per_xcd * kXcds;
+    for (int vt = blockIdx.x; vt < nvirt; vt += gridDim.x) {
     int tile_m, tile_n;
-    if (!xcd_tile(blockIdx.x, tm, tn, per_xcd, tile_m, tile_n)) return;
+    if (!xcd_tile(vt, tm, tn, per_xcd, tile_m, tile_n)) continue;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wid = tid >> 6;
@@ -190,6 +194,8 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
                 *cp = *cp - acc[g][c];
             }
         }
+    }
+    __syncthreads();  // LDS is reused by the next tile
     }
 }
 

@@ -301,6 +301,49 @@ __global__ __launch_bounds__(256) void laswp_rowmap_kernel(T* __restrict__ A, lo
     }
 }
 
+// Applies nmaps consecutive row maps (each the net moves of one inner panel, stride
+// map_stride ints) in order to the columns [c0a, c1a) and [c0b, c1b): one launch for a whole
+// block column's interchanges. Each workgroup moves a strip of kSwapCols columns; the maps
+// are applied one after the other with a block-wide fence between them.
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_multi_kernel(T* __restrict__ A, long long lda,
+    const int* __restrict__ rowmaps, int nmaps, int map_stride, int W2, long long c0a, long long c1a, long long c0b,
+    long long c1b)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* buf = reinterpret_cast<T*>(lds_raw);  // [W2][kSwapCols]
+    const long long la = c1a - c0a;
+    const long long total = la + (c1b - c0b);
+    const long long s = static_cast<long long>(blockIdx.x) * kSwapCols;
+    for (int mi = 0; mi < nmaps; ++mi) {
+        const int* rm = rowmaps + static_cast<size_t>(mi) * map_stride;
+        const int cnt = rm[0];
+        const int* dst = rm + 1;
+        const int* src = rm + 1 + W2;
+        for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
+            const int q = idx / kSwapCols;
+            const int cc = idx - q * kSwapCols;
+            const long long t = s + cc;
+            if (t < total) {
+                const long long col = (t < la) ? c0a + t : c0b + (t - la);
+                buf[idx] = A[static_cast<long long>(src[q]) * lda + col];
+            }
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
+            const int q = idx / kSwapCols;
+            const int cc = idx - q * kSwapCols;
+            const long long t = s + cc;
+            if (t < total) {
+                const long long col = (t < la) ? c0a + t : c0b + (t - la);
+                A[static_cast<long long>(dst[q]) * lda + col] = buf[idx];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
 // U12 = L11^-1 * A12: rows k..k+w-1, columns [c0, c1). One thread per column; L11 (unit
 // lower, from the factored diagonal block) is broadcast from LDS. The rows are processed in
 // chunks of 16 held in registers; earlier chunks are re-read from memory (just written, so

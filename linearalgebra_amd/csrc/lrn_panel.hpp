@@ -20,6 +20,7 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPanel2MaxWG = 64;  // workgroups of one launch
 constexpr int kPanel2CPT = 9;     // chunks read per thread per column: ceil((64 * 32 + 32) / 256)
+inline int panel_cpt(int P) { return P < 8 ? 1 : P < 16 ? 2 : P < 24 ? 3 : P < 32 ? 4 : P < 40 ? 5 : 9; }
 
 struct alignas(16) Chunk {
     double v;
@@ -48,9 +49,65 @@ __device__ __forceinline__ void st_chunk(Chunk* p, double v, int idx, unsigned t
     asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(d) : "memory");
 }
 
-// Nine agent-coherent 16-byte loads in flight at once; the wait is inside the same asm
+// N agent-coherent 16-byte loads in flight at once; the wait is inside the same asm
 // statement, so no output register is live while its load is outstanding.
-__device__ __forceinline__ void ld_chunks9(u32x4 (&r)[kPanel2CPT], const Chunk* const (&p)[kPanel2CPT])
+template <int N> __device__ __forceinline__ void ld_chunks(u32x4 (&r)[N], const Chunk* const (&p)[N]);
+template <> __device__ __forceinline__ void ld_chunks<1>(u32x4 (&r)[1], const Chunk* const (&p)[1])
+{
+    asm volatile(
+        "global_load_dwordx4 %0, %1, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(r[0])
+        : "v"(p[0])
+        : "memory");
+}
+template <> __device__ __forceinline__ void ld_chunks<2>(u32x4 (&r)[2], const Chunk* const (&p)[2])
+{
+    asm volatile(
+        "global_load_dwordx4 %0, %2, off sc1\n\t"
+        "global_load_dwordx4 %1, %3, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1])
+        : "v"(p[0]), "v"(p[1])
+        : "memory");
+}
+template <> __device__ __forceinline__ void ld_chunks<3>(u32x4 (&r)[3], const Chunk* const (&p)[3])
+{
+    asm volatile(
+        "global_load_dwordx4 %0, %3, off sc1\n\t"
+        "global_load_dwordx4 %1, %4, off sc1\n\t"
+        "global_load_dwordx4 %2, %5, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2])
+        : "memory");
+}
+template <> __device__ __forceinline__ void ld_chunks<4>(u32x4 (&r)[4], const Chunk* const (&p)[4])
+{
+    asm volatile(
+        "global_load_dwordx4 %0, %4, off sc1\n\t"
+        "global_load_dwordx4 %1, %5, off sc1\n\t"
+        "global_load_dwordx4 %2, %6, off sc1\n\t"
+        "global_load_dwordx4 %3, %7, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3])
+        : "memory");
+}
+template <> __device__ __forceinline__ void ld_chunks<5>(u32x4 (&r)[5], const Chunk* const (&p)[5])
+{
+    asm volatile(
+        "global_load_dwordx4 %0, %5, off sc1\n\t"
+        "global_load_dwordx4 %1, %6, off sc1\n\t"
+        "global_load_dwordx4 %2, %7, off sc1\n\t"
+        "global_load_dwordx4 %3, %8, off sc1\n\t"
+        "global_load_dwordx4 %4, %9, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4])
+        : "memory");
+}
+template <> __device__ __forceinline__ void ld_chunks<9>(u32x4 (&r)[9], const Chunk* const (&p)[9])
 {
     asm volatile(
         "global_load_dwordx4 %0, %9, off sc1\n\t"
@@ -63,8 +120,7 @@ __device__ __forceinline__ void ld_chunks9(u32x4 (&r)[kPanel2CPT], const Chunk* 
         "global_load_dwordx4 %7, %16, off sc1\n\t"
         "global_load_dwordx4 %8, %17, off sc1\n\t"
         "s_waitcnt vmcnt(0)"
-        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7]),
-          "=&v"(r[8])
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7]), "=&v"(r[8])
         : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]), "v"(p[8])
         : "memory");
 }
@@ -83,12 +139,19 @@ template <typename T, int W> __host__ __device__ constexpr size_t panel2_lds_byt
     return panel2_tile_bytes<T, W>(R) + static_cast<size_t>(P) * W * sizeof(double);
 }
 
-template <typename T, int W>
-__global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restrict__ A, long long lda, long long k,
+// Version with the trailing part of each elimination deferred. Column j+1 is brought up to
+// date right after column j's multipliers are known; the rest of column j's rank-1 update
+// (columns j+2..) is applied during step j+1 after this workgroup has published its
+// candidate, so it overlaps the time the other workgroups' chunks take to arrive. The
+// candidate row and the diagonal row receive that deferred update before they are
+// published. Every element still receives its updates one by one in column order, each
+// product rounded before the subtraction, so the result is that of the plain order.
+// CPT = chunks each thread reads per column (>= ceil((P*W + W) / 256)).
+template <typename T, int W, int CPT>
+__global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restrict__ A, long long lda, long long k,
     long long m, int w, int R, int P, unsigned tag_base, unsigned spin_limit, long long* __restrict__ piv,
     Panel2Ws<T> ws)
 {
-    static_assert(kPanel2MaxWG * W + W <= kPanel2CPT * kPanelThreads, "chunk budget");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     constexpr int LD = W + 1;
     T* tile = reinterpret_cast<T*>(lds_raw);
@@ -98,6 +161,7 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
     __shared__ double red_v[kPanelThreads / kWave];
     __shared__ long long red_i[kPanelThreads / kWave];
     __shared__ long long pivs[W];
+    __shared__ long long win_row;
     __shared__ int win_b;
 
     const int tid = threadIdx.x;
@@ -107,31 +171,55 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
     const long long row0 = k + static_cast<long long>(b) * R;
     const long long n = k + m;
     const int rows_here = static_cast<int>(min(static_cast<long long>(R), n - row0));
+    const int total_chunks = P * w + w;
 
     for (int idx = tid; idx < rows_here * w; idx += kPanelThreads) {
         const int r = idx / w;
         const int c = idx - r * w;
         tile[r * LD + c] = A[(row0 + r) * lda + k + c];
     }
-    __syncthreads();
 
+    // rows of the argmax scan: contiguous per thread, so lane order is row order
+    const int rpt = (R + kPanelThreads - 1) / kPanelThreads;
+    const int r_lo = tid * rpt;
+    const int r_hi = min(r_lo + rpt, rows_here);
+    // elimination mapping: tpr threads per row
     const int tpr = (R >= kPanelThreads) ? 1 : (kPanelThreads / R);
     const int rstep = kPanelThreads / tpr;
-    const int total_chunks = P * w + w;
 
+    // chunk addresses are fixed for the launch except for the parity
+    const Chunk* addr0[CPT];
+    const Chunk* addr1[CPT];
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        int q = tid + s * kPanelThreads;
+        if (q >= total_chunks) q = q % total_chunks;  // spare loads spread over real chunks
+        if (q < P * w) {
+            addr0[s] = &ws.slots[(0 * kPanel2MaxWG + q / w) * W + (q % w)];
+            addr1[s] = &ws.slots[(1 * kPanel2MaxWG + q / w) * W + (q % w)];
+        } else {
+            addr0[s] = &ws.dslots[0 * W + (q - P * w)];
+            addr1[s] = &ws.dslots[1 * W + (q - P * w)];
+        }
+    }
+    __syncthreads();
+
+    __shared__ int prev_zero;  // column j-1 had an exactly-zero pivot: it has no deferred update
+    if (tid == 0) prev_zero = 1;
+    __syncthreads();
     for (int j = 0; j < w; ++j) {
         const long long i = k + j;
         const int par = j & 1;
         const unsigned tag = tag_base + static_cast<unsigned>(j) + 1u;
 
-        // 1. local argmax
+        // (i) local argmax over column j (current)
         double bv = -2.0;
         long long bi = 0x7fffffffffffffffll;
-        for (int r = tid; r < rows_here; r += kPanelThreads) {
+        for (int r = r_lo; r < r_hi; ++r) {
             const long long g = row0 + r;
             if (g >= i) {
                 const double v = pivot_key(tile[r * LD + j], g == i);
-                if (argmax_better(v, g, bv, bi)) {
+                if (v > bv) {
                     bv = v;
                     bi = g;
                 }
@@ -151,38 +239,61 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
                 bv = red_v[q];
                 bi = red_i[q];
             }
+        const int bl = static_cast<int>(bi - row0);  // local index of the candidate row
+
+        const bool deferred = !prev_zero;  // j == 0 starts with prev_zero set
+        // (ii) deferred update of column j-1 on the candidate row and the diagonal row
+        if (deferred) {
+            const double* pu = cand + win_b * W;  // previous pivot row (still in LDS)
+            const int jm = j - 1;
+            if (tid < w && tid > j) {
+                T* rc = tile + bl * LD;
+                rc[tid] = rc[tid] - rc[jm] * static_cast<T>(pu[tid]);
+            }
+            if (b == 0 && tid >= kWave && tid - kWave < w && tid - kWave > j && bl != j) {
+                const int c = tid - kWave;
+                T* rd = tile + j * LD;
+                rd[c] = rd[c] - rd[jm] * static_cast<T>(pu[c]);
+            }
+        }
+        __syncthreads();
 
         long long p;
-        const double* prow;
         if (P > 1) {
-            // 2. publish the candidate row; workgroup 0 also publishes the diagonal row
-            const int bl = static_cast<int>(bi - row0);
+            // (iii) publish
             if (tid < w)
                 st_chunk(&ws.slots[(static_cast<size_t>(par) * kPanel2MaxWG + b) * W + tid],
                     static_cast<double>(tile[bl * LD + tid]), static_cast<int>(bi), tag);
             if (b == 0 && tid >= kWave && tid - kWave < w)
                 st_chunk(&ws.dslots[par * W + (tid - kWave)], static_cast<double>(tile[j * LD + (tid - kWave)]),
                     static_cast<int>(i), tag);
-            // 3. gather every candidate row and the diagonal row, repeating until all tags match
-            const Chunk* addr[kPanel2CPT];
-#pragma unroll
-            for (int s = 0; s < kPanel2CPT; ++s) {
-                const int q = tid + s * kPanelThreads;
-                const Chunk* a = ws.slots;
-                if (q < P * w)
-                    a = &ws.slots[(static_cast<size_t>(par) * kPanel2MaxWG + q / w) * W + (q % w)];
-                else if (q < total_chunks)
-                    a = &ws.dslots[par * W + (q - P * w)];
-                addr[s] = a;
+        }
+
+        // (iv) deferred update of column j-1 on every other row, columns > j
+        if (deferred) {
+            const double* pu = cand + win_b * W;
+            const int jm = j - 1;
+            for (int rr = tid / tpr; rr < rows_here; rr += rstep) {
+                if (row0 + rr < i || rr == bl || (b == 0 && rr == j)) continue;
+                T* rowp = tile + rr * LD;
+                const T l = rowp[jm];
+                for (int c = j + 1 + (tid % tpr); c < w; c += tpr) rowp[c] = rowp[c] - l * static_cast<T>(pu[c]);
             }
-            u32x4 r[kPanel2CPT];
+        }
+
+        if (P > 1) {
+            // (v) gather all candidates and the diagonal row
+            u32x4 r[CPT];
             unsigned spins = 0;
             for (;;) {
-                ld_chunks9(r, addr);
+                if (par)
+                    ld_chunks<CPT>(r, addr1);
+                else
+                    ld_chunks<CPT>(r, addr0);
                 bool ok = true;
 #pragma unroll
-                for (int s = 0; s < kPanel2CPT; ++s)
-                    if (tid + s * kPanelThreads < total_chunks && r[s].w != tag) ok = false;
+                for (int s = 0; s < CPT; ++s)
+                    if (r[s].w != tag) ok = false;
                 if (ok) break;
                 ++spins;
                 if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(ws.err))) {
@@ -191,8 +302,9 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __syncthreads();  // every thread is done with cand / win_b of column j-1
 #pragma unroll
-            for (int s = 0; s < kPanel2CPT; ++s) {
+            for (int s = 0; s < CPT; ++s) {
                 const int q = tid + s * kPanelThreads;
                 if (q < P * w) {
                     const int bb = q / w, c = q % w;
@@ -203,42 +315,35 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
                 }
             }
             __syncthreads();
-            // 4. global winner over the P candidates
-            double gv = -2.0;
-            long long gi = 0x7fffffffffffffffll;
-            if (tid < P) {
-                gi = cand_idx[tid];
-                gv = pivot_key(cand[tid * W + j], gi == i);
-            }
-            wave_argmax(gv, gi);
-            if (lane == 0) {
-                red_v[wid] = gv;
-                red_i[wid] = gi;
-            }
-            __syncthreads();
-            gv = red_v[0];
-            gi = red_i[0];
-#pragma unroll
-            for (int q = 1; q < kPanelThreads / kWave; ++q)
-                if (argmax_better(red_v[q], red_i[q], gv, gi)) {
-                    gv = red_v[q];
-                    gi = red_i[q];
+            // (vi) winner over the P candidates (one wave)
+            if (wid == 0) {
+                double gv = -2.0;
+                long long gi = 0x7fffffffffffffffll;
+                if (lane < P) {
+                    gi = cand_idx[lane];
+                    gv = pivot_key(cand[lane * W + j], gi == i);
                 }
-            p = gi;
-            if (tid == 0) win_b = static_cast<int>((p - k) / R);
+                wave_argmax(gv, gi);
+                if (lane == 0) {
+                    win_row = gi;
+                    win_b = static_cast<int>((gi - k) / R);
+                }
+            }
             __syncthreads();
-            prow = cand + win_b * W;
+            p = win_row;
         } else {
-            p = bi;
+            __syncthreads();
             if (tid < w) {
-                cand[tid] = static_cast<double>(tile[static_cast<int>(p - row0) * LD + tid]);
+                cand[tid] = static_cast<double>(tile[bl * LD + tid]);
                 drow[tid] = static_cast<double>(tile[j * LD + tid]);
             }
+            if (tid == 0) win_b = 0;
             __syncthreads();
-            prow = cand;
+            p = bi;
         }
+        const double* prow = cand + win_b * W;
 
-        // 5. interchange rows i and p inside the panel columns
+        // (vii) interchange rows i and p inside the panel columns
         if (p != i) {
             if (b == 0 && tid < w) tile[j * LD + tid] = static_cast<T>(prow[tid]);
             const long long pl = p - row0;
@@ -250,19 +355,22 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
         }
         __syncthreads();
 
-        // 6. multipliers and rank-1 update below the diagonal
+        // (viii) multipliers of column j and the update of column j+1 only
         const T d = static_cast<T>(prow[j]);
+        if (tid == 0) prev_zero = (d == T(0)) ? 1 : 0;
         if (d == T(0)) {
+            // exactly-zero pivot: the column is skipped (no multipliers, no update)
             if (b == 0 && tid == 0 && *ws.info == 0) *ws.info = static_cast<int>(i + 1);
         } else {
             const bool use_recip = fabs(static_cast<double>(d)) >= static_cast<double>(TypeMin<T>::v);
             const T rcp = use_recip ? T(1) / d : T(1);
-            for (int rr = tid / tpr; rr < rows_here; rr += rstep) {
+            const T u1 = (j + 1 < w) ? static_cast<T>(prow[j + 1]) : T(0);
+            for (int rr = tid; rr < rows_here; rr += kPanelThreads) {
                 if (row0 + rr <= i) continue;
                 T* rowp = tile + rr * LD;
                 const T l = use_recip ? rowp[j] * rcp : rowp[j] / d;
-                for (int c = j + 1 + (tid % tpr); c < w; c += tpr) rowp[c] = rowp[c] - l * static_cast<T>(prow[c]);
-                if ((tid % tpr) == 0) rowp[j] = l;
+                rowp[j] = l;
+                if (j + 1 < w) rowp[j + 1] = rowp[j + 1] - l * u1;
             }
         }
         __syncthreads();
@@ -277,3 +385,4 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel2_kernel(T* __restri
 }
 
 }  // namespace lrn
+
