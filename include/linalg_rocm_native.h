@@ -100,6 +100,33 @@ LRN_API int lrn_getrf_async_f64(double* m, int64_t lda, int64_t* pivot, int* inf
 LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
     size_t workspace_size, void* stream);
 
+/* ---- mixed-precision solve with iterative refinement (no reference counterpart) --------
+ * Solves A x = b (FP64 A and b, one right-hand side) to FP64-level normwise backward error
+ * with a low-precision factorisation: A is rounded to FP32, factored with FP32 panels and
+ * bf16 matrix-core trailing updates accumulated in FP32; refinement then runs with the
+ * residual r = b - A x, the solution update and the GMRES inner products in FP64.
+ * LU-IR first; when it stops converging, GMRES-IR (GMRES preconditioned by the
+ * low-precision factors, restart `gmres_restart`); if that fails too and the workspace was
+ * sized with fallback = 1, a plain FP64 LU solve. A is not modified. SYNCHRONOUS.
+ * Convergence: ||b - A x||_inf <= tol * (||A||_inf ||x||_inf + ||b||_inf). */
+typedef struct lrn_mixed_stats {
+    int info;              /* zero pivot of the low-precision factorisation (1-based), 0 = none */
+    int path;              /* 0 LU-IR, 1 GMRES-IR, 2 FP64 fallback, 3 not converged */
+    int ir_iterations;     /* solves with the low-precision factors during LU-IR */
+    int gmres_outer;       /* GMRES-IR refinement steps */
+    int gmres_iterations;  /* inner GMRES iterations, all restarts */
+    double backward_error; /* final normwise backward error (the convergence measure above) */
+    double factor_ms;      /* conversion + low-precision factorisation, device time */
+    double refine_ms;      /* refinement (and fallback), device time */
+} lrn_mixed_stats;
+
+/* Trailing update of the low-precision factorisation: split bf16 (head + tail, three bf16
+ * products, the default: split = 1) or plain bf16 (split = 0). Process-wide. */
+LRN_API int lrn_set_mixed_update(int split);
+LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int fallback, size_t* bytes);
+LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,
+    int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream);
+
 /* Device tuning knobs (outer block, panel width, panel workgroups) for the autotuner. A
  * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);
@@ -110,7 +137,9 @@ LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg);
 LRN_API int lrn_set_lookahead(int enable);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
- * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32.
+ * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,
+ * 2 = f32 data with bf16 operands and FP32 accumulation, 3 = the same with head + tail split
+ * operands (three bf16 products) (gemm_sub only).
  * gemm_sub: C[M x N] -= A[M x K] * B[K x N] (row-major, leading dims lda/ldb/ldc).
  * trsm: rows k..k+w-1, columns [c0, c1) of the lda x lda matrix a <- L11^-1 a, with L11 the
  *       unit-lower w x w block at (k, k).

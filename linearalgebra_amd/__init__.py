@@ -13,4 +13,6 @@ from .lu import (  # noqa: F401
     set_tuning,
 )
 
-__all__ = ["lu_decomposition", "linear_solver", "linear_solver_type", "LinalgError", "set_tuning", "get_tuning"]
+from .mixed import gesv_mixed  # noqa: F401,E402
+
+__all__ = ["gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type", "LinalgError", "set_tuning", "get_tuning"]

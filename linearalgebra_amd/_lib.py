@@ -38,6 +38,9 @@ SIGNATURES = {
     "lrn_debug_panel": [_int, _vp, _i64, _i64, _int, _vp, _vp, _sz, _vp],
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
+    "lrn_set_mixed_update": [_int],
+    "lrn_gesv_mixed_workspace_size_f64": [_i64, _int, _int, ctypes.POINTER(_sz)],
+    "lrn_gesv_mixed_f64": [_vp, _i64, _vp, _vp, ctypes.c_double, _int, _int, _vp, _sz, _vp, _vp],
     "lrn_get_tuning": [ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)],
 }
 _RESTYPES = {"lrn_status_string": ctypes.c_char_p, "lrn_version": ctypes.c_char_p}

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
@@ -28,6 +29,8 @@
 #include "lrn_getrs_kernels.hpp"
 #include "lrn_panel.hpp"
 #include "lrn_gemm_f64.hpp"
+#include "lrn_gemm_bf16.hpp"
+#include "lrn_mixed.hpp"
 
 namespace lrn {
 namespace {
@@ -370,6 +373,46 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
     }
 }
 
+std::atomic<int> g_bf16_split{1};
+
+// C[M x N] -= bf16 products of A and B, FP32 data (trailing update of the mixed-precision LU):
+// split (head + tail, three products) unless switched off with lrn_set_mixed_update(0)
+int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long long ldb, float* C, long long ldc,
+    long long M, long long N, long long K, hipStream_t s, int split = -1)
+{
+    const bool sp = split < 0 ? g_bf16_split.load() != 0 : split != 0;
+    if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
+    const long long tm = (M + kBfBM - 1) / kBfBM, tn = (N + kBfBN - 1) / kBfBN;
+    if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
+    const long long per_xcd = (tm * tn + 7) / 8;
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    const long long cap = 4ll * device_props(dev).cus;
+    const dim3 grid(static_cast<unsigned>(std::min(per_xcd * 8, cap)));
+    if (sp)
+        hipLaunchKernelGGL(gemm_sub_bf16_kernel<true>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, static_cast<int>(M),
+            static_cast<int>(N), static_cast<int>(K), static_cast<int>(tm), static_cast<int>(tn), static_cast<int>(per_xcd));
+    else
+        hipLaunchKernelGGL(gemm_sub_bf16_kernel<false>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc,
+            static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tm), static_cast<int>(tn),
+            static_cast<int>(per_xcd));
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+// The large trailing update: FP64 / FP32 matrix-core GEMM, or bf16 operands with FP32
+// accumulation for the low-precision factorisation of the mixed solver.
+template <typename T>
+int launch_trailing(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M, long long N,
+    long long K, bool lowp, hipStream_t s)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (lowp) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s);
+    }
+    (void)lowp;
+    return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s);
+}
+
 int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
 {
     if (count <= 0) return LRN_OK;
@@ -491,7 +534,7 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
 // The panel and GEMM streams are restricted to disjoint sets of compute units so the panel's
 // cooperative workgroups always find room. Nothing waits on the host.
 template <typename T, int W>
-int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s)
+int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, bool lowp = false)
 {
     const long long lda = n;
     const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
@@ -514,8 +557,8 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
             LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, KB, n, s));
             if (KB >= n) break;
             LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
-            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
-                n - KB, B, s));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
+                n - KB, B, lowp, s));
             LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s)));
             continue;
         }
@@ -526,8 +569,8 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, far0, n, side->gemm));
         if (far0 < n) {
             LRN_TRY((u12_columns<T, W>(A, n, K, B, far0, n, side->gemm)));
-            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
-                n - far0, B, side->gemm));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
+                n - far0, B, lowp, side->gemm));
         }
         LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
         // next block column: caller stream
@@ -535,8 +578,8 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         if (B2 > 0) {
             LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, s));
             LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, s)));
-            LRN_TRY(launch_gemm_sub<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
-                B, s));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
+                B, lowp, s));
             LRN_HIP(hipEventRecord(side->ready, s));
             LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
             LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel)));
@@ -580,9 +623,8 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
 
 // Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
 // without touching B (LU mode after a singular factorisation).
-template <typename T>
-int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nrhs, long long ldb,
-    const GetrsLayout<T>& L, const int* skip_dev, hipStream_t s)
+// Row permutation of the factorisation (from the pivots), into L.perm.
+template <typename T> int getrs_perm(long long n, const long long* piv, const GetrsLayout<T>& L, hipStream_t s)
 {
     const size_t perm_lds = (n <= kPermLdsMax) ? static_cast<size_t>(n) * sizeof(int) : 0;
     {
@@ -591,6 +633,15 @@ int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nr
     }
     hipLaunchKernelGGL(perm_from_pivots_kernel, dim3(1), dim3(64), perm_lds, s, piv, n, L.perm, L.perm_tmp);
     LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+// Forward and backward substitution with the permutation already in L.perm.
+// skip_dev: if non-null and nonzero, the kernels return without touching B.
+template <typename T>
+int getrs_subst(const T* LU, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& L,
+    const int* skip_dev, hipStream_t s)
+{
     const long long nblk = solve_blocks(n);
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
@@ -605,6 +656,15 @@ int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nr
         LRN_HIP(hipGetLastError());
     }
     return LRN_OK;
+}
+
+// Solve with factors already in place.
+template <typename T>
+int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nrhs, long long ldb,
+    const GetrsLayout<T>& L, const int* skip_dev, hipStream_t s)
+{
+    LRN_TRY(getrs_perm<T>(n, piv, L, s));
+    return getrs_subst<T>(LU, n, B, nrhs, ldb, L, skip_dev, s);
 }
 
 constexpr long long kMaxOrder = 1ll << 17;
@@ -684,6 +744,314 @@ int debug_panel_impl(T* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* 
     unsigned base = 0;
     if (w <= 32) return launch_panel<T, 32>(a, lda, k, lda - k, w, reinterpret_cast<long long*>(pivot), L.ws, base, s);
     return launch_panel<T, kMaxW>(a, lda, k, lda - k, w, reinterpret_cast<long long*>(pivot), L.ws, base, s);
+}
+
+
+// ------------------------------------------------------------------- mixed-precision solve
+
+struct MixedLayout {
+    float* a32;
+    long long* piv;
+    GetrfLayout<float> f;
+    GetrsLayout<float> g;
+    double* r;
+    double* w;
+    double* d;
+    double* xbest;
+    double* rowsum;
+    float* r32;
+    double* V;     // (m + 1) vectors of length n
+    double* h1;
+    double* h2;
+    double* scal;  // 8 scalars
+    double* afb;   // FP64 copy for the fallback (null when not sized)
+    long long* pivfb;
+    GetrfLayout<double> ffb;
+    GetrsLayout<double> gfb;
+};
+
+MixedLayout carve_mixed(Arena& ar, long long n, int m, bool fallback)
+{
+    MixedLayout L;
+    L.a32 = ar.next<float>(static_cast<size_t>(n) * static_cast<size_t>(n));
+    L.piv = ar.next<long long>(static_cast<size_t>(n));
+    L.f = carve_getrf<float>(ar);
+    L.g = carve_getrs<float>(ar, n, 1);
+    L.r = ar.next<double>(static_cast<size_t>(n));
+    L.w = ar.next<double>(static_cast<size_t>(n));
+    L.d = ar.next<double>(static_cast<size_t>(n));
+    L.xbest = ar.next<double>(static_cast<size_t>(n));
+    L.rowsum = ar.next<double>(static_cast<size_t>(n));
+    L.r32 = ar.next<float>(static_cast<size_t>(n));
+    L.V = ar.next<double>(static_cast<size_t>(m + 1) * static_cast<size_t>(n));
+    L.h1 = ar.next<double>(static_cast<size_t>(m + 2));
+    L.h2 = ar.next<double>(static_cast<size_t>(m + 2));
+    L.scal = ar.next<double>(8);
+    L.afb = nullptr;
+    L.pivfb = nullptr;
+    if (fallback) {
+        L.afb = ar.next<double>(static_cast<size_t>(n) * static_cast<size_t>(n));
+        L.pivfb = ar.next<long long>(static_cast<size_t>(n));
+        L.ffb = carve_getrf<double>(ar);
+        L.gfb = carve_getrs<double>(ar, n, 1);
+    }
+    return L;
+}
+
+size_t mixed_ws_bytes(long long n, int m, bool fallback)
+{
+    Arena ar;
+    carve_mixed(ar, n, m, fallback);
+    return ar.off;
+}
+
+unsigned grid_for(long long n, int threads = 256)
+{
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>((n + threads - 1) / threads, 1 << 20)));
+}
+
+struct MixedSolver {
+    const double* A;
+    long long n;
+    const double* b;
+    double* x;
+    const MixedLayout& L;
+    hipStream_t s;
+    double anorm = 0.0, bnorm = 0.0;
+
+    int host_scalar(const double* dev, double* out)
+    {
+        LRN_HIP(hipMemcpyAsync(out, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+        LRN_HIP(hipStreamSynchronize(s));
+        return LRN_OK;
+    }
+    int reduce(const double* u, const double* v, int op, double* out)
+    {
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, u, v, n, op, L.scal);
+        LRN_HIP(hipGetLastError());
+        return host_scalar(L.scal, out);
+    }
+    // out = beta * bb + alpha * A v
+    int gemv(const double* v, const double* bb, double* out, double alpha, double beta)
+    {
+        hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, v, bb, out,
+            alpha, beta, 0);
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
+    // out (FP64) = M^-1 in (FP64), with M the low-precision factors; in == out allowed
+    int precond(const double* in, double* out)
+    {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, in, L.r32, n);
+        LRN_HIP(hipGetLastError());
+        LRN_TRY(getrs_subst<float>(L.a32, n, L.r32, 1, 1, L.g, nullptr, s));
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, s, L.r32, out, n);
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
+    // r = b - A x and the backward error
+    int residual(double* berr)
+    {
+        LRN_TRY(gemv(x, b, L.r, -1.0, 1.0));
+        double rn = 0.0, xn = 0.0;
+        LRN_TRY(reduce(L.r, nullptr, 0, &rn));
+        LRN_TRY(reduce(x, nullptr, 0, &xn));
+        const double den = anorm * xn + bnorm;
+        *berr = den > 0.0 ? rn / den : rn;
+        return LRN_OK;
+    }
+    int axpby(double* y, const double* xx, double alpha, double beta)
+    {
+        hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, xx, n, alpha, beta, nullptr, nullptr);
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
+    int copy(double* dst, const double* src)
+    {
+        LRN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, s));
+        return LRN_OK;
+    }
+
+    // One restart cycle of left-preconditioned GMRES on A d = r (d returned in L.d), classical
+    // Gram-Schmidt with one re-orthogonalisation, FP64 vectors and inner products.
+    int gmres(int m, double rtol, int* iters)
+    {
+        const long long ldv = n;
+        LRN_TRY(precond(L.r, L.w));
+        double beta2 = 0.0;
+        LRN_TRY(reduce(L.w, nullptr, 2, &beta2));
+        const double beta = std::sqrt(beta2);
+        LRN_TRY(axpby(L.d, nullptr, 0.0, 0.0));
+        if (!(beta > 0.0)) return LRN_OK;
+        std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
+        g[0] = beta;
+        // v0 = w / beta
+        hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, L.V, L.w, n, 0.0, 1.0 / beta, nullptr,
+            nullptr);
+        LRN_HIP(hipGetLastError());
+        int j = 0;
+        std::vector<double> h(m + 2), h2(m + 2);
+        for (; j < m; ++j) {
+            double* vj = L.V + static_cast<long long>(j) * ldv;
+            LRN_TRY(gemv(vj, nullptr, L.w, 1.0, 0.0));
+            LRN_TRY(precond(L.w, L.w));
+            for (int pass = 0; pass < 2; ++pass) {
+                double* hd = pass == 0 ? L.h1 : L.h2;
+                hipLaunchKernelGGL(multi_dot_kernel, dim3(j + 1), dim3(256), 0, s, L.V, ldv, L.w, n, hd);
+                LRN_HIP(hipGetLastError());
+                hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s, L.V, ldv, hd, j + 1, L.w, n);
+                LRN_HIP(hipGetLastError());
+            }
+            LRN_HIP(hipMemcpyAsync(h.data(), L.h1, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+            LRN_HIP(hipMemcpyAsync(h2.data(), L.h2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+            double wn2 = 0.0;
+            LRN_TRY(reduce(L.w, nullptr, 2, &wn2));  // synchronises the copies above too
+            for (int i = 0; i <= j; ++i) h[i] += h2[i];
+            h[j + 1] = std::sqrt(wn2);
+            // apply the previous rotations, then a new one
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = t;
+            }
+            const double rr = std::hypot(h[j], h[j + 1]);
+            cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+            sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
+            h[j] = rr;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = h[i];
+            const double hn = std::sqrt(wn2);
+            ++*iters;
+            if (hn > 0.0 && j + 1 < m + 1) {
+                hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, L.V + static_cast<long long>(j + 1) * ldv,
+                    L.w, n, 0.0, 1.0 / hn, nullptr, nullptr);
+                LRN_HIP(hipGetLastError());
+            }
+            if (std::fabs(g[j + 1]) <= rtol * beta || !(hn > 0.0)) {
+                ++j;
+                break;
+            }
+        }
+        // y = H(0:j, 0:j)^-1 g(0:j); d = V y
+        std::vector<double> y(j, 0.0);
+        for (int i = j - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < j; ++k) t -= H[static_cast<size_t>(i) * m + k] * y[k];
+            y[i] = t / H[static_cast<size_t>(i) * m + i];
+        }
+        for (auto& v : y) v = -v;
+        if (j > 0) {
+            LRN_HIP(hipMemcpyAsync(L.h1, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s, L.V, ldv, L.h1, j, L.d, n);
+            LRN_HIP(hipGetLastError());
+            LRN_HIP(hipStreamSynchronize(s));  // y (host memory) must outlive the copy
+        }
+        return LRN_OK;
+    }
+};
+
+int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, double tol, int max_iter, int m,
+    void* ws, size_t ws_size, lrn_mixed_stats* stats, hipStream_t s)
+{
+    const bool fallback = ws_size >= mixed_ws_bytes(n, m, true);
+    if (ws_size < mixed_ws_bytes(n, m, false)) return -9;
+    Arena ar(ws, ws_size);
+    const MixedLayout L = carve_mixed(ar, n, m, fallback);
+    lrn_mixed_stats st{};
+    st.path = 3;
+    hipEvent_t e0, e1, e2;
+    LRN_HIP(hipEventCreate(&e0));
+    LRN_HIP(hipEventCreate(&e1));
+    LRN_HIP(hipEventCreate(&e2));
+    struct EvGuard {
+        hipEvent_t a, b, c;
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            (void)hipEventDestroy(c);
+        }
+    } guard{e0, e1, e2};
+    LRN_HIP(hipEventRecord(e0, s));
+    const long long nn = n * n;
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(nn)), dim3(256), 0, s, A, L.a32, nn);
+    LRN_HIP(hipGetLastError());
+    LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, true)));
+    LRN_TRY(getrs_perm<float>(n, L.piv, L.g, s));
+    LRN_HIP(hipEventRecord(e1, s));
+    int info = 0, err = 0;
+    LRN_HIP(hipMemcpyAsync(&info, L.f.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipMemcpyAsync(&err, L.f.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipStreamSynchronize(s));
+    if (err != 0) return LRN_E_TIMEOUT;
+    st.info = info;
+
+    MixedSolver S{A, n, b, x, L, s};
+    hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, nullptr,
+        nullptr, L.rowsum, 0.0, 0.0, 1);
+    LRN_HIP(hipGetLastError());
+    LRN_TRY(S.reduce(L.rowsum, nullptr, 0, &S.anorm));
+    LRN_TRY(S.reduce(b, nullptr, 0, &S.bnorm));
+
+    double berr = 1.0 / 0.0;
+    bool done = false;
+    if (info == 0) {
+        // LU-IR: x = M^-1 b, then x += M^-1 (b - A x) while the backward error keeps falling fast
+        LRN_TRY(S.axpby(x, nullptr, 0.0, 0.0));
+        LRN_TRY(S.copy(L.r, b));
+        double best = 1.0 / 0.0;
+        for (int it = 0; it < max_iter; ++it) {
+            LRN_TRY(S.precond(L.r, L.d));
+            LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
+            ++st.ir_iterations;
+            LRN_TRY(S.residual(&berr));
+            if (berr < best) {
+                LRN_TRY(S.copy(L.xbest, x));
+                best = berr;
+            }
+            if (berr <= tol) {
+                st.path = 0;
+                done = true;
+                break;
+            }
+            if (!(berr <= 0.9 * best) && it >= 2) break;  // stagnating: switch to GMRES-IR
+            if (!(berr == berr) || berr > 1e6 * best) break;
+        }
+        if (!done) {
+            // restart GMRES-IR from the best LU-IR iterate
+            LRN_TRY(S.copy(x, L.xbest));
+            LRN_TRY(S.residual(&berr));
+            for (int outer = 0; outer < max_iter && !done; ++outer) {
+                LRN_TRY(S.gmres(m, 1e-10, &st.gmres_iterations));
+                LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
+                ++st.gmres_outer;
+                LRN_TRY(S.residual(&berr));
+                if (berr <= tol) {
+                    st.path = 1;
+                    done = true;
+                }
+            }
+        }
+    }
+    if (!done && fallback) {
+        LRN_HIP(hipMemcpyAsync(L.afb, A, sizeof(double) * static_cast<size_t>(nn), hipMemcpyDeviceToDevice, s));
+        LRN_TRY(getrf_dispatch<double>(L.afb, n, L.pivfb, L.ffb.ws, s));
+        LRN_TRY(S.copy(x, b));
+        LRN_TRY(getrs_run<double>(L.afb, n, L.pivfb, x, 1, 1, L.gfb, L.ffb.ws.info, s));
+        LRN_TRY(S.residual(&berr));
+        st.path = 2;
+    }
+    LRN_HIP(hipEventRecord(e2, s));
+    LRN_HIP(hipEventSynchronize(e2));
+    float f_ms = 0.f, r_ms = 0.f;
+    LRN_HIP(hipEventElapsedTime(&f_ms, e0, e1));
+    LRN_HIP(hipEventElapsedTime(&r_ms, e1, e2));
+    st.factor_ms = f_ms;
+    st.refine_ms = r_ms;
+    st.backward_error = berr;
+    if (stats) *stats = st;
+    return LRN_OK;
 }
 
 }  // namespace
@@ -806,6 +1174,9 @@ LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void
     if (dtype == 1)
         return launch_gemm_sub<float>(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
             static_cast<float*>(c), ldc, m, n, k, s);
+    if (dtype == 2 || dtype == 3)
+        return launch_gemm_sub_bf16(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
+            static_cast<float*>(c), ldc, m, n, k, s, dtype == 3 ? 1 : 0);
     return -1;
 }
 
@@ -827,6 +1198,37 @@ LRN_API int lrn_debug_panel(int dtype, void* a, int64_t lda, int64_t k, int w, i
     if (dtype == 0) return debug_panel_impl<double>(static_cast<double*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
     if (dtype == 1) return debug_panel_impl<float>(static_cast<float*>(a), lda, k, w, pivot, workspace, workspace_size, stream);
     return -1;
+}
+
+LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int fallback, size_t* bytes)
+{
+    if (n <= 0) return -1;
+    if (gmres_restart <= 0 || gmres_restart > 200) return -2;
+    if (bytes == nullptr) return -4;
+    *bytes = mixed_ws_bytes(n, gmres_restart, fallback != 0);
+    return LRN_OK;
+}
+
+LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,
+    int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream)
+{
+    if (a == nullptr) return -1;
+    if (n <= 0) return -2;
+    if (b == nullptr) return -3;
+    if (x == nullptr) return -4;
+    if (!(tol > 0.0)) return -5;
+    if (max_iter <= 0) return -6;
+    if (gmres_restart <= 0 || gmres_restart > 200) return -7;
+    if (workspace == nullptr) return -8;
+    if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
+    return gesv_mixed_impl(a, n, b, x, tol, max_iter, gmres_restart, workspace, workspace_size, stats,
+        static_cast<hipStream_t>(stream));
+}
+
+LRN_API int lrn_set_mixed_update(int split)
+{
+    g_bf16_split.store(split ? 1 : 0);
+    return LRN_OK;
 }
 
 LRN_API int lrn_set_lookahead(int enable)

@@ -1,0 +1,135 @@
+// Vector and matrix-vector kernels of the mixed-precision solver (iterative refinement and
+// GMRES-IR): precision conversions, the FP64 residual / product with A, norms and inner
+// products. Every reduction has a fixed order (one workgroup per result), so results are
+// reproducible run to run.
+#pragma once
+
+#include "lrn_device.hpp"
+
+namespace lrn {
+
+template <typename Tin, typename Tout>
+__global__ void convert_kernel(const Tin* __restrict__ in, Tout* __restrict__ out, long long count)
+{
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        out[i] = static_cast<Tout>(in[i]);
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// out[i] = beta * b[i] + alpha * (A x)[i] (b may be null: treated as 0), one wave per row,
+// FP64 with fused multiply-adds. mode 1 instead stores sum_j |A[i][j]| (for ||A||_inf).
+__global__ __launch_bounds__(256) void gemv_rows_kernel(const double* __restrict__ A, long long n,
+    const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ out, double alpha, double beta,
+    int mode)
+{
+    const int lane = threadIdx.x & 63;
+    const long long row = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* a = A + row * n;
+    double s = 0.0;
+    if (mode == 0) {
+        for (long long c = lane; c < n; c += 64) s = fma(a[c], x[c], s);
+    } else {
+        for (long long c = lane; c < n; c += 64) s += fabs(a[c]);
+    }
+    s = wave_sum_f64(s);
+    if (lane == 0) {
+        if (mode == 0)
+            out[row] = (b != nullptr ? beta * b[row] : 0.0) + alpha * s;
+        else
+            out[row] = s;
+    }
+}
+
+// Single-workgroup reductions of length-n vectors (deterministic order).
+// op 0: max |u|; op 1: sum u*v; op 2: sum u*u.
+__global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ u, const double* __restrict__ v,
+    long long n, int op, double* __restrict__ out)
+{
+    __shared__ double part[16];
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024) {
+        const double a = u[i];
+        if (op == 0)
+            s = fmax(s, fabs(a));
+        else if (op == 1)
+            s = fma(a, v[i], s);
+        else
+            s = fma(a, a, s);
+    }
+    if (op == 0) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, 64));
+    } else {
+        s = wave_sum_f64(s);
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = part[0];
+        for (int q = 1; q < 16; ++q) t = (op == 0) ? fmax(t, part[q]) : t + part[q];
+        *out = t;
+    }
+}
+
+// h[k] = sum_i V[k][i] * w[i] for k < nv (one workgroup per k); V holds vectors of length n
+// with stride ldv.
+__global__ __launch_bounds__(256) void multi_dot_kernel(const double* __restrict__ V, long long ldv,
+    const double* __restrict__ w, long long n, double* __restrict__ h)
+{
+    __shared__ double part[4];
+    const double* v = V + static_cast<long long>(blockIdx.x) * ldv;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) s = fma(v[i], w[i], s);
+    s = wave_sum_f64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) h[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// w[i] -= sum_k V[k][i] * h[k], k < nv
+__global__ void multi_axpy_kernel(const double* __restrict__ V, long long ldv, const double* __restrict__ h, int nv,
+    double* __restrict__ w, long long n)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = w[i];
+    for (int k = 0; k < nv; ++k) s = fma(-V[static_cast<long long>(k) * ldv + i], h[k], s);
+    w[i] = s;
+}
+
+// y = alpha * y + beta * x (alpha / beta from host values, or from device scalars when the
+// pointers are non-null: y = y * (*pa) + x * (*pb))
+__global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, long long n, double alpha,
+    double beta, const double* __restrict__ pa, const double* __restrict__ pb)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = pa ? *pa : alpha;
+    const double b = pb ? *pb : beta;
+    y[i] = a * y[i] + (x ? b * x[i] : 0.0);
+}
+
+// y += (double) d  (FP32 correction added to the FP64 iterate)
+__global__ void add_f32_kernel(double* __restrict__ y, const float* __restrict__ d, long long n)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += static_cast<double>(d[i]);
+}
+
+// v = w / s with s a device scalar (GMRES normalisation)
+__global__ void scale_inv_kernel(double* __restrict__ v, const double* __restrict__ w, long long n,
+    const double* __restrict__ s)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = w[i] / *s;
+}
+
+}  // namespace lrn

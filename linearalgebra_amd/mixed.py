@@ -1,0 +1,66 @@
+"""Mixed-precision FP64 solve (build-owned extension of linear_solver, no reference counterpart).
+
+A is rounded to FP32 and factored with FP32 panels and bf16 matrix-core trailing updates
+(FP32 accumulation); LU-IR, then GMRES-IR, then (optionally) a plain FP64 LU recover an
+FP64-level normwise backward error. All compute is in liblinalg_rocm_native.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .lu import LinalgError, _check, _ptr, _stream_ptr
+
+PATHS = {0: "lu-ir", 1: "gmres-ir", 2: "fp64-fallback", 3: "not-converged"}
+
+
+class MixedStats(ctypes.Structure):
+    _fields_ = [
+        ("info", ctypes.c_int),
+        ("path", ctypes.c_int),
+        ("ir_iterations", ctypes.c_int),
+        ("gmres_outer", ctypes.c_int),
+        ("gmres_iterations", ctypes.c_int),
+        ("backward_error", ctypes.c_double),
+        ("factor_ms", ctypes.c_double),
+        ("refine_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["path_name"] = PATHS.get(self.path, "?")
+        return d
+
+
+def workspace_size(n: int, restart: int = 30, fallback: bool = True) -> int:
+    out = ctypes.c_size_t(0)
+    _check(_lib.load().lrn_gesv_mixed_workspace_size_f64(int(n), int(restart), int(bool(fallback)), ctypes.byref(out)),
+           "gesv_mixed::workspace_size")
+    return int(out.value)
+
+
+def gesv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, tol: float = 1e-14,
+               max_iter: int = 20, restart: int = 30, workspace: torch.Tensor | None = None,
+               fallback: bool = True, stream=None):
+    """Solve a x = b (FP64, one right-hand side). Returns (x, stats dict). ``a`` is not modified."""
+    if a is None or b is None:
+        raise ValueError("gesv_mixed: a and b must not be null")
+    n = a.shape[0]
+    if a.dtype != torch.float64 or b.dtype != torch.float64 or not a.is_cuda or not b.is_cuda:
+        raise ValueError("gesv_mixed: a and b must be float64 device tensors")
+    if not a.is_contiguous() or a.numel() != n * n or b.numel() < n:
+        raise ValueError("gesv_mixed: a must be a contiguous n x n matrix and b a vector of n")
+    if x is None:
+        x = torch.empty(n, dtype=torch.float64, device=a.device)
+    if workspace is None:
+        workspace = torch.empty(workspace_size(n, restart, fallback), dtype=torch.uint8, device=a.device)
+    st = MixedStats()
+    code = _lib.load().lrn_gesv_mixed_f64(_ptr(a), int(n), _ptr(b), _ptr(x), float(tol), int(max_iter), int(restart),
+                                          _ptr(workspace), workspace.numel(), ctypes.byref(st), _stream_ptr(stream))
+    _check(code, "gesv_mixed")
+    return x, st.as_dict()
+
+
+__all__ = ["gesv_mixed", "workspace_size", "MixedStats", "LinalgError"]
