@@ -1,0 +1,327 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 LU solve path: FP64-accurate solves of A x = b at n = 16384.
+
+One step = one complete solve of a resident n x n FP64 system (nrhs = 1):
+  mixed  (the headline `value`): lrn_gesv_mixed_f64 -- FP32 panels, split-bf16 MFMA trailing
+         update with FP32 accumulation, LU-IR then GMRES-IR with FP64 residuals, to a
+         normwise backward error <= 1e-14;
+  plain  (reported beside it): lrn_linear_solver_f64 in LU mode -- the FP64 blocked LU with
+         partial pivoting (FP64 4x4x4 MFMA trailing update) and the triangular solves.
+
+Input: A = U diag(s) V^T with U, V Haar-orthogonal (Q of the QR of a seeded Gaussian, sign
+fixed by R's diagonal) and s log-spaced from 1 to 1/kappa; b iid N(0, 1). Generated on the
+GPU with torch (not timed). The timed region starts with A and b resident in HBM.
+
+Live roofline: the library's kernel trace (HIP events recorded on the launch stream around
+every launch of the traced kernel kinds) runs during the timed steps; `roofline` is the
+dominant matrix-core kernel of the measured path (algorithmic flops 2*M*N*K per launch,
+summed over launches, divided by the summed launch durations). `phases` gives every kernel
+kind from one extra traced step. `cpu_baseline` is the CPU restatement of the reference's
+native algorithm timed on this host on a smaller order and scaled by n^3.
+
+Usage: python bench.py [--steps K] [--warmup W] [--n N] [--kappa K] [--cpu-n N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+METRIC = "FP64-accurate Ax=b solves/s: mixed LU-IR vs plain FP64 LU, n=16384; batch solves/s"
+PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
+TRACE = {"gemm_f64": 0, "gemm_f64_edge": 1, "gemm_bf16x3": 2, "gemm_bf16": 3, "gemm_f32": 4, "panel": 5,
+         "laswp": 6, "trsm": 7, "trsv": 8}
+FLOP_KINDS = {"gemm_f64", "gemm_f64_edge", "gemm_bf16x3", "gemm_bf16", "gemm_f32", "panel"}
+KERNEL_SYMBOL = {"gemm_f64": "gemm_sub_f64_kernel<false>", "gemm_bf16x3": "gemm_sub_bf16_kernel<true>"}
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--kappa", type=float, default=1e6)
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--restart", type=int, default=30)
+    p.add_argument("--cpu-n", type=int, default=2048, help="order of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    return p.parse_args()
+
+
+# ----------------------------------------------------------------------------- trace helpers
+
+def trace_start(lib, kinds, capacity=1 << 16):
+    mask = 0
+    for k in kinds:
+        mask |= 1 << TRACE[k]
+    assert lib.lrn_trace_start(mask, capacity) == 0
+
+
+def trace_read(lib, kind):
+    c, ms, w = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    assert lib.lrn_trace_read(TRACE[kind], ctypes.byref(c), ctypes.byref(ms), ctypes.byref(w)) == 0
+    return int(c.value), float(ms.value), float(w.value)
+
+
+def phase_table(lib):
+    out = {}
+    for k in TRACE:
+        c, ms, w = trace_read(lib, k)
+        if c == 0:
+            continue
+        e = {"launches": c, "ms": round(ms, 3), "avg_us": round(1e3 * ms / c, 2)}
+        if k in FLOP_KINDS:
+            e["tflops"] = round(w / (ms * 1e-3) / 1e12, 2) if ms > 0 else None
+        else:
+            e["gbs"] = round(w / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
+        out[k] = e
+    return out
+
+
+def roofline(lib, kind, peak_tflops, traffic):
+    c, ms, w = trace_read(lib, kind)
+    if c == 0 or ms <= 0:
+        return None
+    achieved = w / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": KERNEL_SYMBOL.get(kind, kind), "achieved": round(achieved, 2),
+            "peak": peak_tflops, "unit": "TFLOP/s", "frac": round(achieved / peak_tflops, 4), "traffic": traffic,
+            "launches": c, "avg_launch_us": round(1e3 * ms / c, 2),
+            "flops_per_launch": round(w / c, 1)}
+
+
+def traffic_from_profiles(kind):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy-kernel calibration) and written into the committed summary; None if absent."""
+    path = os.path.join(HERE, "profiles", f"{kind}_n16384_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+# ----------------------------------------------------------------------------- inputs
+
+def conditioned_gpu(torch, n, kappa, seed, dev):
+    """U diag(s) V^T on the GPU, U and V Haar (Q of a seeded Gaussian, R's diagonal made positive)."""
+    g = torch.Generator(device=dev)
+
+    def haar(sd):
+        g.manual_seed(sd)
+        m = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        q, r = torch.linalg.qr(m)
+        del m
+        q *= torch.sign(torch.diagonal(r)).unsqueeze(0)
+        del r
+        return q
+
+    u = haar(seed)
+    s = torch.logspace(0.0, -torch.log10(torch.tensor(kappa)).item(), n, dtype=torch.float64, device=dev)
+    u *= s.unsqueeze(0)
+    v = haar(seed + 1)
+    a = u @ v.T
+    del u, v
+    g.manual_seed(seed + 2)
+    b = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    return a.contiguous(), b
+
+
+def backward_error_gpu(torch, a, x, b):
+    r = b - a @ x
+    den = torch.linalg.matrix_norm(a, ord=float("inf")) * x.abs().max() + b.abs().max()
+    return float(r.abs().max() / den)
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+
+def cpu_baseline(n_cpu, n_target, seed):
+    """The CPU restatement of the reference's native LU + lu_solve (single thread, as the
+    reference is single-threaded), timed at n_cpu and scaled to n_target by n^3."""
+    sys.path.insert(0, HERE)
+    import numpy as np
+    from oracle import oracle as O
+
+    O.build()
+    a = O.gaussian(n_cpu, seed=seed)
+    b = O.gaussian(n_cpu, 1, seed=seed + 1)[:, 0]
+    t0 = time.perf_counter()
+    _, _, x, info = O.gesv(a, b, threads=1)
+    dt = time.perf_counter() - t0
+    assert info == 0 and np.isfinite(x).all()
+    scale = (n_target / n_cpu) ** 3
+    res = {"value": round(1.0 / (dt * scale), 6), "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": f"oracle getrf+getrs (FP64, partial pivoting, N(0,1)) at n={n_cpu}, nrhs=1, 1 thread: "
+                     f"{dt:.2f} s; scaled by (n/{n_cpu})^3 to n={n_target}"}
+    # vendor LAPACK beside it (scipy's OpenBLAS), all host cores
+    try:
+        import scipy.linalg as sl
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        lu, piv = sl.lu_factor(a, check_finite=False)
+        sl.lu_solve((lu, piv), b, check_finite=False)
+        dl = time.perf_counter() - t0
+        res["lapack"] = {"value": round(1.0 / (dl * scale), 6), "unit": "solves/s", "cores": cores,
+                         "sample": f"scipy lu_factor+lu_solve (OpenBLAS dgetrf/dgetrs) at n={n_cpu}, "
+                                   f"{cores} host cores: {dl:.3f} s; scaled by n^3 to n={n_target}"}
+    except Exception as e:  # noqa: BLE001 - optional extra
+        res["lapack"] = f"unavailable: {e}"
+    return res
+
+
+# ----------------------------------------------------------------------------- main
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        sys.exit(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+                 "the multi-GPU layout of the factorisation is a design, not an implementation")
+    import torch
+
+    sys.path.insert(0, HERE)
+    import linearalgebra_amd as la
+    from linearalgebra_amd import _lib
+    from linearalgebra_amd import mixed as lam
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    n, steps, warmup = args.n, args.steps, args.warmup
+    log(f"generating A (n={n}, kappa={args.kappa:g}) on the GPU")
+    t0 = time.perf_counter()
+    a, b = conditioned_gpu(torch, n, args.kappa, args.seed, dev)
+    torch.cuda.synchronize()
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
+
+    # ---------------- mixed LU-IR (headline)
+    ws = torch.empty(lam.workspace_size(n, args.restart, True), dtype=torch.uint8, device=dev)
+    x = torch.empty(n, dtype=torch.float64, device=dev)
+    for i in range(warmup):
+        _, st = la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=args.restart, workspace=ws)
+        log(f"mixed warmup {i}: path={st['path']} ir={st['ir_iterations']} gmres={st['gmres_iterations']} "
+            f"berr={st['backward_error']:.2e} factor={st['factor_ms']:.1f} ms refine={st['refine_ms']:.1f} ms")
+    trace_start(lib, ["gemm_bf16x3"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        _, st = la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=args.restart, workspace=ws)
+        stats.append(st)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    lib.lrn_trace_stop()
+    roof_mixed = roofline(lib, "gemm_bf16x3", PEAK["bf16_mfma_tflops"], traffic_from_profiles("gemm_bf16x3"))
+    log(f"mixed: {1e3 * t_mixed / steps:.1f} ms/solve")
+    berr = backward_error_gpu(torch, a, x, b)
+    trace_start(lib, list(TRACE))
+    la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=args.restart, workspace=ws)
+    torch.cuda.synchronize()
+    lib.lrn_trace_stop()
+    phases_mixed = phase_table(lib)
+    x_mixed = x.clone()
+    del ws
+
+    # ---------------- plain FP64 LU + solve
+    plain = None
+    roof_fp64 = None
+    phases_plain = None
+    fwd = None
+    if not args.no_plain:
+        aw = torch.empty_like(a)
+        piv = torch.empty(n, dtype=torch.int64, device=dev)
+        xp = torch.empty(n, dtype=torch.float64, device=dev)
+        wsp = torch.empty(la.linear_solver.workspace_size(n, la.linear_solver_type.LU_LINEAR_SOLVER),
+                          dtype=torch.uint8, device=dev)
+
+        def plain_step():
+            aw.copy_(a)
+            xp.copy_(b)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            la.linear_solver.apply(aw, piv, n, xp, la.linear_solver_type.LU_LINEAR_SOLVER, wsp)
+            e1.record()
+            return e0, e1
+
+        for _ in range(warmup):
+            plain_step()
+        torch.cuda.synchronize()
+        trace_start(lib, ["gemm_f64"])
+        evs = [plain_step() for _ in range(steps)]
+        torch.cuda.synchronize()
+        lib.lrn_trace_stop()
+        t_plain = sum(e0.elapsed_time(e1) for e0, e1 in evs) * 1e-3
+        roof_fp64 = roofline(lib, "gemm_f64", PEAK["fp64_mfma_tflops"], traffic_from_profiles("gemm_f64"))
+        log(f"plain FP64: {1e3 * t_plain / steps:.1f} ms/solve")
+        berr_plain = backward_error_gpu(torch, a, xp, b)
+        fwd = float((x_mixed - xp).abs().max() / xp.abs().max())
+        trace_start(lib, list(TRACE))
+        plain_step()
+        torch.cuda.synchronize()
+        lib.lrn_trace_stop()
+        phases_plain = phase_table(lib)
+        plain = {"solves_per_s": round(steps / t_plain, 4), "ms_per_solve": round(1e3 * t_plain / steps, 2),
+                 "backward_error": berr_plain,
+                 "timing": "HIP events around lrn_linear_solver_f64 (LU mode) on the launch stream; the "
+                           "restore copy of A before each solve is outside the events",
+                 "lu_tflops": round(2.0 * n ** 3 / 3.0 / (t_plain / steps) / 1e12, 2)}
+        del aw, wsp
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline at n={args.cpu_n}")
+        cpu = cpu_baseline(args.cpu_n, n, args.seed)
+
+    last = stats[-1]
+    value = steps / t_mixed
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(1e3 * t_mixed / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 panels + split-bf16 MFMA update (fp32 accumulate), f64 residual/refinement",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision LU-IR solve of A x = b, n={n}, nrhs=1, "
+                               f"A = U diag(s) V^T (Haar U, V; s log-spaced 1..1/kappa), kappa={args.kappa:g}, "
+                               f"b iid N(0,1), backward error <= 1e-14",
+                   "n": n, "nrhs": 1, "kappa": args.kappa, "seed": args.seed, "parallelism": "single GPU"},
+        "backward_error": berr,
+        "library_backward_error": last["backward_error"],
+        "path": ["lu-ir", "gmres-ir", "fp64-fallback", "not-converged"][last["path"]],
+        "ir_iterations": last["ir_iterations"],
+        "gmres_iterations": last["gmres_iterations"],
+        "factor_ms": round(last["factor_ms"], 2),
+        "refine_ms": round(last["refine_ms"], 2),
+        "forward_error_vs_fp64_lu": fwd,
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": round(value / plain["solves_per_s"], 3) if plain else None,
+        "roofline": roof_mixed,
+        "roofline_fp64_lu": roof_fp64,
+        "phases_mixed": phases_mixed,
+        "phases_plain_fp64": phases_plain,
+        "cpu_baseline": cpu,
+    }
+    if not (berr <= 1e-14):
+        line["invalid"] = f"backward error {berr:.3e} above 1e-14"
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -120,9 +120,12 @@ typedef struct lrn_mixed_stats {
     double refine_ms;      /* refinement (and fallback), device time */
 } lrn_mixed_stats;
 
-/* Trailing update of the low-precision factorisation: split bf16 (head + tail, three bf16
- * products, the default: split = 1) or plain bf16 (split = 0). Process-wide. */
-LRN_API int lrn_set_mixed_update(int split);
+/* Trailing update of the low-precision factorisation (process-wide): every FP32 operand is
+ * split into bf16 planes x ~ p0 + p1 + p2 and the plane products run on the bf16 matrix
+ * cores with FP32 accumulation. mode 0 = one plane (bf16 operands), 1 = two planes (three
+ * products, ~2^-16 relative), 2 = three planes (six products, FP32-level factors; the
+ * default), 3 = FP32 matrix instructions. */
+LRN_API int lrn_set_mixed_update(int mode);
 LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int fallback, size_t* bytes);
 LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,
     int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream);
@@ -138,8 +141,7 @@ LRN_API int lrn_set_lookahead(int enable);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,
- * 2 = f32 data with bf16 operands and FP32 accumulation, 3 = the same with head + tail split
- * operands (three bf16 products) (gemm_sub only).
+ * 2, 3, 4 = f32 data split into 1, 2, 3 bf16 planes, FP32 accumulation (gemm_sub only).
  * gemm_sub: C[M x N] -= A[M x K] * B[K x N] (row-major, leading dims lda/ldb/ldc).
  * trsm: rows k..k+w-1, columns [c0, c1) of the lda x lda matrix a <- L11^-1 a, with L11 the
  *       unit-lower w x w block at (k, k).
@@ -150,6 +152,30 @@ LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void
 LRN_API int lrn_debug_trsm(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t c0, int64_t c1, void* stream);
 LRN_API int lrn_debug_panel(int dtype, void* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* workspace,
     size_t workspace_size, void* stream);
+
+/* ---- kernel timing trace (no reference counterpart) ------------------------------------
+ * While a trace runs, each launch of a traced kernel kind is bracketed by two HIP events on
+ * the stream it is launched on, and its algorithmic work is logged. `kinds` is a bit mask
+ * of LRN_TRACE_* values; `capacity` bounds the number of traced launches (events are
+ * created by lrn_trace_start, outside any timed region; launches beyond it go untimed).
+ * lrn_trace_read synchronises on the events of one kind and returns the number of
+ * launches, their summed duration (ms) and summed work (flops for GEMM and panel kinds,
+ * bytes for the others). Process-wide; for measurement, not for concurrent use. */
+enum lrn_trace_kind {
+    LRN_TRACE_GEMM_F64 = 0,        /* FP64 trailing GEMM, interior tiles (flops 2MNK) */
+    LRN_TRACE_GEMM_F64_EDGE = 1,   /* FP64 trailing GEMM, ragged strips (flops) */
+    LRN_TRACE_GEMM_BF16X3 = 2,     /* two-plane bf16 trailing GEMM (FP32-equivalent flops 2MNK) */
+    LRN_TRACE_GEMM_BF16 = 3,       /* one-plane bf16 trailing GEMM (flops) */
+    LRN_TRACE_GEMM_F32 = 4,        /* FP32 GEMM (flops) */
+    LRN_TRACE_PANEL = 5,           /* panel factorisation (flops) */
+    LRN_TRACE_LASWP = 6,           /* row interchanges (bytes, every pivot taken as a swap) */
+    LRN_TRACE_TRSM = 7,            /* U12 triangular solve (bytes) */
+    LRN_TRACE_TRSV = 8,            /* solve substitutions (bytes) */
+    LRN_TRACE_GEMM_BF16X6 = 9      /* three-plane bf16 trailing GEMM (FP32-equivalent flops 2MNK) */
+};
+LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity);
+LRN_API int lrn_trace_stop(void);
+LRN_API int lrn_trace_read(int kind, int64_t* launches, double* milliseconds, double* work);
 
 /* Version string of this library. */
 LRN_API const char* lrn_version(void);

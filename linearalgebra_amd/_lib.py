@@ -39,6 +39,9 @@ SIGNATURES = {
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
     "lrn_set_mixed_update": [_int],
+    "lrn_trace_start": [ctypes.c_uint, _i64],
+    "lrn_trace_stop": [],
+    "lrn_trace_read": [_int, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lrn_gesv_mixed_workspace_size_f64": [_i64, _int, _int, ctypes.POINTER(_sz)],
     "lrn_gesv_mixed_f64": [_vp, _i64, _vp, _vp, ctypes.c_double, _int, _int, _vp, _sz, _vp, _vp],
     "lrn_get_tuning": [ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)],

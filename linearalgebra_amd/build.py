@@ -36,12 +36,31 @@ def _run(cmd):
     return out
 
 
+CXX_LIB = os.path.join(HERE, "liblinalg_rocm_native_cxx.so")
+CXX_TEST = os.path.join(ROOT, "tests", "cxx", "test_rocm_native_backend")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
+
+
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, "lrn_capi.hip")]
-    out = _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB, *srcs])
+    """The HIP library, then the C++ ROCM_NATIVE specialisations linked against it."""
+    inc = ["-I", os.path.join(ROOT, "include")]
+    out = _run([HIPCC, *HIP_FLAGS, *inc, "-o", LIB, os.path.join(CSRC, "lrn_capi.hip")])
+    out += _run(["g++", *CXX_FLAGS, "-shared", *inc, "-o", CXX_LIB, os.path.join(CSRC, "rocm_native_backend.cpp"),
+                 "-L", HERE, "-llinalg_rocm_native", "-Wl,-rpath,$ORIGIN"])
     if verbose and out.strip():
         print(out)
     return LIB
+
+
+def build_cxx_test() -> str:
+    """The C++ backend test (host code; HIP runtime for device buffers, the oracle library as
+    the checker). Needs the oracle built first (oracle/Makefile)."""
+    oracle_dir = os.path.join(ROOT, "oracle")
+    src = CXX_TEST + ".cpp"
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", CXX_TEST, src,
+          "-L", HERE, "-llinalg_rocm_native_cxx", "-llinalg_rocm_native", "-L", oracle_dir, "-llu_oracle",
+          f"-Wl,-rpath,{HERE}:{oracle_dir}:$ORIGIN/../../linearalgebra_amd:$ORIGIN/../../oracle"])
+    return CXX_TEST
 
 
 if __name__ == "__main__":

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -31,6 +32,7 @@
 #include "lrn_gemm_f64.hpp"
 #include "lrn_gemm_bf16.hpp"
 #include "lrn_mixed.hpp"
+#include "lrn_trace.hpp"
 
 namespace lrn {
 namespace {
@@ -209,6 +211,14 @@ int g_panel2_target_wg()
     return (v > 0) ? std::min(v, kPanel2MaxWG) : 32;
 }
 
+// flops of the rank-1 updates and scalings of an m x w panel
+double panel_flops(long long m, int w)
+{
+    double f = 0.0;
+    for (int j = 0; j < w && j < m; ++j) f += static_cast<double>(m - j - 1) * (1.0 + 2.0 * (w - j - 1));
+    return f;
+}
+
 template <typename T, int W>
 int launch_panel(T* A, long long lda, long long k, long long m, int w, long long* piv, const PanelWs<T>& ws,
     unsigned& base, hipStream_t s)
@@ -241,6 +251,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
                 int per_cu = 0;
                 LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
                 if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
+                TraceScope ts(kTrPanel, panel_flops(m, w), s);
                 hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit,
                     piv, pw);
                 LRN_HIP(hipGetLastError());
@@ -278,6 +289,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
         LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
         if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
     }
+    TraceScope ts(kTrPanel, panel_flops(m, w), s);
     hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv, ws);
     LRN_HIP(hipGetLastError());
     if (P > 1) base += static_cast<unsigned>(P) * static_cast<unsigned>(w);
@@ -292,6 +304,8 @@ int launch_laswp(T* A, long long lda, const int* rowmap, int W2, long long c0a, 
     if (total <= 0) return LRN_OK;
     const long long blocks = (total + kSwapCols - 1) / kSwapCols;
     const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
+    // bytes if every pivot of the panel is an interchange (W2 / 2 swaps, 2 rows read + written each)
+    TraceScope ts(kTrLaswp, 2.0 * W2 * static_cast<double>(total) * sizeof(T), s);
     hipLaunchKernelGGL(laswp_rowmap_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmap,
         W2, c0a, c1a, c0b, c1b);
     LRN_HIP(hipGetLastError());
@@ -308,6 +322,7 @@ int launch_laswp_multi(T* A, long long lda, const int* rowmaps, int nmaps, int m
     const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
     int per_cu = 0;
     LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(laswp_multi_kernel<T>), lds, 256, &per_cu));
+    TraceScope ts(kTrLaswp, 2.0 * W2 * nmaps * static_cast<double>(total) * sizeof(T), s);
     hipLaunchKernelGGL(laswp_multi_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmaps,
         nmaps, map_stride, W2, c0a, c1a, c0b, c1b);
     LRN_HIP(hipGetLastError());
@@ -319,6 +334,7 @@ int launch_trsm(T* A, long long lda, long long k, int w, long long c0, long long
 {
     if (c1 <= c0) return LRN_OK;
     const long long blocks = (c1 - c0 + 255) / 256;
+    TraceScope ts(kTrTrsm, (static_cast<double>(w) * w + 2.0 * w * static_cast<double>(c1 - c0)) * sizeof(T), s);
     hipLaunchKernelGGL((trsm_llnu_kernel<T, W>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, k, w, c0,
         c1);
     LRN_HIP(hipGetLastError());
@@ -347,6 +363,7 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             LRN_HIP(hipGetDevice(&dev));
             const long long cap = 2ll * device_props(dev).cus;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
+            TraceScope ts(check ? kTrGemmF64Edge : kTrGemmF64, 2.0 * m * nn * K, s);
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<true>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
@@ -366,6 +383,7 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
         constexpr int BM = 128, BN = 128, BK = 16;
         const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
         if (tm * tn > 0x7fffffffll) return LRN_E_UNSUPPORTED;
+        TraceScope ts(kTrGemmF32, 2.0 * M * N * K, s);
         hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s, A,
             lda, B, ldb, C, ldc, static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tn));
         LRN_HIP(hipGetLastError());
@@ -373,15 +391,16 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
     }
 }
 
-std::atomic<int> g_bf16_split{1};
+// Trailing update of the low-precision factorisation: 0 = bf16 operands, 1 = two bf16 planes
+// (three products), 2 = three bf16 planes (six products, FP32-level), 3 = FP32 MFMA.
+std::atomic<int> g_mixed_update{2};
 
-// C[M x N] -= bf16 products of A and B, FP32 data (trailing update of the mixed-precision LU):
-// split (head + tail, three products) unless switched off with lrn_set_mixed_update(0)
+// C[M x N] -= A * B, FP32 data, with `planes` bf16 planes per operand (1..3)
 int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long long ldb, float* C, long long ldc,
-    long long M, long long N, long long K, hipStream_t s, int split = -1)
+    long long M, long long N, long long K, hipStream_t s, int planes)
 {
-    const bool sp = split < 0 ? g_bf16_split.load() != 0 : split != 0;
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
+    if (planes < 1 || planes > 3) return LRN_E_INTERNAL;
     const long long tm = (M + kBfBM - 1) / kBfBM, tn = (N + kBfBN - 1) / kBfBN;
     if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
     const long long per_xcd = (tm * tn + 7) / 8;
@@ -389,13 +408,16 @@ int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long lon
     LRN_HIP(hipGetDevice(&dev));
     const long long cap = 4ll * device_props(dev).cus;
     const dim3 grid(static_cast<unsigned>(std::min(per_xcd * 8, cap)));
-    if (sp)
-        hipLaunchKernelGGL(gemm_sub_bf16_kernel<true>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, static_cast<int>(M),
-            static_cast<int>(N), static_cast<int>(K), static_cast<int>(tm), static_cast<int>(tn), static_cast<int>(per_xcd));
+    const int kind = planes == 1 ? kTrGemmBf16 : (planes == 2 ? kTrGemmBf16x3 : kTrGemmBf16x6);
+    TraceScope ts(kind, 2.0 * M * N * K, s);
+    const int iM = static_cast<int>(M), iN = static_cast<int>(N), iK = static_cast<int>(K);
+    const int itm = static_cast<int>(tm), itn = static_cast<int>(tn), ipx = static_cast<int>(per_xcd);
+    if (planes == 1)
+        hipLaunchKernelGGL(gemm_sub_bf16_kernel<1>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, iM, iN, iK, itm, itn, ipx);
+    else if (planes == 2)
+        hipLaunchKernelGGL(gemm_sub_bf16_kernel<2>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, iM, iN, iK, itm, itn, ipx);
     else
-        hipLaunchKernelGGL(gemm_sub_bf16_kernel<false>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc,
-            static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tm), static_cast<int>(tn),
-            static_cast<int>(per_xcd));
+        hipLaunchKernelGGL(gemm_sub_bf16_kernel<3>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, iM, iN, iK, itm, itn, ipx);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -407,7 +429,8 @@ int launch_trailing(const T* A, long long lda, const T* B, long long ldb, T* C, 
     long long K, bool lowp, hipStream_t s)
 {
     if constexpr (std::is_same<T, float>::value) {
-        if (lowp) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s);
+        const int mode = g_mixed_update.load();
+        if (lowp && mode <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, mode + 1);
     }
     (void)lowp;
     return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s);
@@ -636,10 +659,11 @@ template <typename T> int getrs_perm(long long n, const long long* piv, const Ge
     return LRN_OK;
 }
 
-// Forward and backward substitution with the permutation already in L.perm.
+// Forward and backward substitution with the permutation already in L.perm; factors of type
+// TF, right-hand sides and arithmetic of type T.
 // skip_dev: if non-null and nonzero, the kernels return without touching B.
-template <typename T>
-int getrs_subst(const T* LU, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& L,
+template <typename T, typename TF = T>
+int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& L,
     const int* skip_dev, hipStream_t s)
 {
     const long long nblk = solve_blocks(n);
@@ -648,10 +672,12 @@ int getrs_subst(const T* LU, long long n, T* B, long long nrhs, long long ldb, c
         LRN_TRY(launch_fill(L.flags_f, 0u, nblk, s));
         LRN_TRY(launch_fill(L.flags_b, 0u, nblk, s));
         LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
-        hipLaunchKernelGGL((trsv_blocked_kernel<T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
+        const double tri_bytes = (0.5 * static_cast<double>(n) * n + 2.0 * n * nc) * sizeof(T);
+        TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
+        hipLaunchKernelGGL((trsv_blocked_kernel<TF, T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
             L.perm, B + c0, ldb, L.y, static_cast<long long>(nc), nc, L.flags_f, kSpinLimit, L.err, skip_dev);
         LRN_HIP(hipGetLastError());
-        hipLaunchKernelGGL((trsv_blocked_kernel<T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
+        hipLaunchKernelGGL((trsv_blocked_kernel<TF, T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
             L.perm, L.y, static_cast<long long>(nc), B + c0, ldb, nc, L.flags_b, kSpinLimit, L.err, skip_dev);
         LRN_HIP(hipGetLastError());
     }
@@ -753,13 +779,12 @@ struct MixedLayout {
     float* a32;
     long long* piv;
     GetrfLayout<float> f;
-    GetrsLayout<float> g;
+    GetrsLayout<double> g;  // FP64 substitution buffers (the FP32 factors are applied in FP64)
     double* r;
     double* w;
     double* d;
     double* xbest;
     double* rowsum;
-    float* r32;
     double* V;     // (m + 1) vectors of length n
     double* h1;
     double* h2;
@@ -776,13 +801,12 @@ MixedLayout carve_mixed(Arena& ar, long long n, int m, bool fallback)
     L.a32 = ar.next<float>(static_cast<size_t>(n) * static_cast<size_t>(n));
     L.piv = ar.next<long long>(static_cast<size_t>(n));
     L.f = carve_getrf<float>(ar);
-    L.g = carve_getrs<float>(ar, n, 1);
+    L.g = carve_getrs<double>(ar, n, 1);
     L.r = ar.next<double>(static_cast<size_t>(n));
     L.w = ar.next<double>(static_cast<size_t>(n));
     L.d = ar.next<double>(static_cast<size_t>(n));
     L.xbest = ar.next<double>(static_cast<size_t>(n));
     L.rowsum = ar.next<double>(static_cast<size_t>(n));
-    L.r32 = ar.next<float>(static_cast<size_t>(n));
     L.V = ar.next<double>(static_cast<size_t>(m + 1) * static_cast<size_t>(n));
     L.h1 = ar.next<double>(static_cast<size_t>(m + 2));
     L.h2 = ar.next<double>(static_cast<size_t>(m + 2));
@@ -839,15 +863,12 @@ struct MixedSolver {
         LRN_HIP(hipGetLastError());
         return LRN_OK;
     }
-    // out (FP64) = M^-1 in (FP64), with M the low-precision factors; in == out allowed
+    // out = M^-1 in, with M the low-precision factors applied in FP64 arithmetic (the factor
+    // entries are widened as they are read); in == out allowed
     int precond(const double* in, double* out)
     {
-        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, in, L.r32, n);
-        LRN_HIP(hipGetLastError());
-        LRN_TRY(getrs_subst<float>(L.a32, n, L.r32, 1, 1, L.g, nullptr, s));
-        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, s, L.r32, out, n);
-        LRN_HIP(hipGetLastError());
-        return LRN_OK;
+        if (in != out) LRN_TRY(copy(out, in));
+        return getrs_subst<double, float>(L.a32, n, out, 1, 1, L.g, nullptr, s);
     }
     // r = b - A x and the backward error
     int residual(double* berr)
@@ -951,6 +972,17 @@ struct MixedSolver {
     }
 };
 
+// LRN_MIXED_VERBOSE=1 in the environment prints the backward error of every refinement
+// step to stderr (diagnostics only).
+bool mixed_verbose()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("LRN_MIXED_VERBOSE");
+        return e != nullptr && e[0] == '1';
+    }();
+    return v;
+}
+
 int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, double tol, int max_iter, int m,
     void* ws, size_t ws_size, lrn_mixed_stats* stats, hipStream_t s)
 {
@@ -978,7 +1010,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(nn)), dim3(256), 0, s, A, L.a32, nn);
     LRN_HIP(hipGetLastError());
     LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, true)));
-    LRN_TRY(getrs_perm<float>(n, L.piv, L.g, s));
+    LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
     LRN_HIP(hipEventRecord(e1, s));
     int info = 0, err = 0;
     LRN_HIP(hipMemcpyAsync(&info, L.f.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1006,6 +1038,8 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
             LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
             ++st.ir_iterations;
             LRN_TRY(S.residual(&berr));
+            if (mixed_verbose()) std::fprintf(stderr, "lrn mixed n=%lld lu-ir %d berr %.3e\n", n, it, berr);
+            const double prev_best = best;
             if (berr < best) {
                 LRN_TRY(S.copy(L.xbest, x));
                 best = berr;
@@ -1015,8 +1049,8 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
                 done = true;
                 break;
             }
-            if (!(berr <= 0.9 * best) && it >= 2) break;  // stagnating: switch to GMRES-IR
-            if (!(berr == berr) || berr > 1e6 * best) break;
+            // stagnating (less than a 2x gain over the best so far) or diverging: GMRES-IR
+            if (it >= 1 && !(berr <= 0.5 * prev_best)) break;
         }
         if (!done) {
             // restart GMRES-IR from the best LU-IR iterate
@@ -1027,6 +1061,9 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
                 LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
                 ++st.gmres_outer;
                 LRN_TRY(S.residual(&berr));
+                if (mixed_verbose())
+                    std::fprintf(stderr, "lrn mixed n=%lld gmres-ir %d (inner %d) berr %.3e\n", n, outer,
+                        st.gmres_iterations, berr);
                 if (berr <= tol) {
                     st.path = 1;
                     done = true;
@@ -1174,9 +1211,9 @@ LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void
     if (dtype == 1)
         return launch_gemm_sub<float>(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
             static_cast<float*>(c), ldc, m, n, k, s);
-    if (dtype == 2 || dtype == 3)
+    if (dtype >= 2 && dtype <= 4)
         return launch_gemm_sub_bf16(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
-            static_cast<float*>(c), ldc, m, n, k, s, dtype == 3 ? 1 : 0);
+            static_cast<float*>(c), ldc, m, n, k, s, dtype - 1);
     return -1;
 }
 
@@ -1225,9 +1262,32 @@ LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, doub
         static_cast<hipStream_t>(stream));
 }
 
-LRN_API int lrn_set_mixed_update(int split)
+LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity)
 {
-    g_bf16_split.store(split ? 1 : 0);
+    if (capacity <= 0 || capacity > (1ll << 22)) return -2;
+    return trace_start(kinds, static_cast<long>(capacity)) == 0 ? LRN_OK : LRN_E_HIP;
+}
+
+LRN_API int lrn_trace_stop(void)
+{
+    trace_stop();
+    return LRN_OK;
+}
+
+LRN_API int lrn_trace_read(int kind, int64_t* launches, double* milliseconds, double* work)
+{
+    if (kind < 0 || kind >= kTrKinds) return -1;
+    if (launches == nullptr || milliseconds == nullptr || work == nullptr) return -2;
+    long long c = 0;
+    if (trace_read(kind, &c, milliseconds, work) != 0) return LRN_E_HIP;
+    *launches = c;
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_mixed_update(int mode)
+{
+    if (mode < 0 || mode > 3) return -1;
+    g_mixed_update.store(mode);
     return LRN_OK;
 }
 

@@ -1,10 +1,11 @@
 // Low-precision trailing update of the mixed-precision factorisation on the bf16 matrix
-// cores, A, B, C FP32 row-major, FP32 accumulation:
-//   kSplit = false: C -= bf16(A) * bf16(B)
-//   kSplit = true : every operand is split into a bf16 head and a bf16 tail (x = h + t, t =
-//                   bf16(x - h)) and C -= Ah*Bh + Ah*Bt + At*Bh, three bf16 products that
-//                   recover ~16 significant bits per product (the dropped At*Bt term is
-//                   ~2^-16 relative), which makes the factors a far better preconditioner.
+// cores, A, B, C FP32 row-major, FP32 accumulation. Every operand x is split into NP bf16
+// planes, x ~ p0 + p1 + p2 with p0 = bf16(x), p1 = bf16(x - p0), p2 = bf16(x - p0 - p1), and
+// C -= sum of the plane products that matter at that accuracy:
+//   NP = 1: A0 B0                                   (bf16 operands, ~2^-8 relative)
+//   NP = 2: A0 B0 + A0 B1 + A1 B0                   (~2^-16)
+//   NP = 3: the NP = 2 terms + A0 B2 + A2 B0 + A1 B1 (~2^-24, FP32-level factors)
+// Products are accumulated smallest first.
 //
 // v_mfma_f32_16x16x32_bf16 (gfx950): lane l supplies A[l % 16][8*(l/16) .. 8*(l/16)+7] and
 // B[8*(l/16) .. +7][l % 16] as 8 packed bf16; the FP32 result element v of lane l is
@@ -23,12 +24,12 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kBfBM = 128, kBfBN = 128, kBfBK = 32, kBfPad = 8;
 
-template <bool kSplit>
+template <int NP>
 __global__ __launch_bounds__(256) void gemm_sub_bf16_kernel(const float* __restrict__ A, long long lda,
     const float* __restrict__ B, long long ldb, float* __restrict__ C, long long ldc, int M, int N, int K, int tm,
     int tn, int per_xcd)
 {
-    constexpr int NP = kSplit ? 2 : 1;  // head (and tail) planes
+    static_assert(NP >= 1 && NP <= 3, "one to three bf16 planes");
     __shared__ __align__(16) __bf16 As[NP][2][kBfBM][kBfBK + kBfPad];
     __shared__ __align__(16) __bf16 Bs[NP][2][kBfBN][kBfBK + kBfPad];
 
@@ -73,27 +74,31 @@ __global__ __launch_bounds__(256) void gemm_sub_bf16_kernel(const float* __restr
             }
         };
         auto store = [&](int buf) {
-            bf16x8_t v0, v1, t0, t1;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                v0[e] = static_cast<__bf16>(ra[e]);
-                v1[e] = static_cast<__bf16>(ra[8 + e]);
-                if (kSplit) {
-                    t0[e] = static_cast<__bf16>(ra[e] - static_cast<float>(v0[e]));
-                    t1[e] = static_cast<__bf16>(ra[8 + e] - static_cast<float>(v1[e]));
+            for (int h = 0; h < 2; ++h) {
+                float rem[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) rem[e] = ra[8 * h + e];
+#pragma unroll
+                for (int pl = 0; pl < NP; ++pl) {
+                    bf16x8_t v;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        v[e] = static_cast<__bf16>(rem[e]);
+                        rem[e] = rem[e] - static_cast<float>(v[e]);
+                    }
+                    *reinterpret_cast<bf16x8_t*>(&As[pl][buf][ar][ak + 8 * h]) = v;
                 }
-            }
-            *reinterpret_cast<bf16x8_t*>(&As[0][buf][ar][ak]) = v0;
-            *reinterpret_cast<bf16x8_t*>(&As[0][buf][ar][ak + 8]) = v1;
-            if (kSplit) {
-                *reinterpret_cast<bf16x8_t*>(&As[NP - 1][buf][ar][ak]) = t0;
-                *reinterpret_cast<bf16x8_t*>(&As[NP - 1][buf][ar][ak + 8]) = t1;
             }
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                const __bf16 h = static_cast<__bf16>(rb[e]);
-                Bs[0][buf][bj + e][bk] = h;
-                if (kSplit) Bs[NP - 1][buf][bj + e][bk] = static_cast<__bf16>(rb[e] - static_cast<float>(h));
+                float rem = rb[e];
+#pragma unroll
+                for (int pl = 0; pl < NP; ++pl) {
+                    const __bf16 h = static_cast<__bf16>(rem);
+                    Bs[pl][buf][bj + e][bk] = h;
+                    rem = rem - static_cast<float>(h);
+                }
             }
         };
 
@@ -104,37 +109,31 @@ __global__ __launch_bounds__(256) void gemm_sub_bf16_kernel(const float* __restr
         for (int s = 0; s < nk; ++s) {
             const int buf = s & 1;
             if (s + 1 < nk) load((s + 1) * kBfBK);
-            bf16x8_t a[4], b[4];
+            bf16x8_t a[NP][4], b[NP][4];
 #pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-                a[mi] = *reinterpret_cast<const bf16x8_t*>(&As[0][buf][wm * 64 + mi * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-            for (int ni = 0; ni < 4; ++ni)
-                b[ni] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][buf][wn * 64 + ni * 16 + (lane & 15)][8 * (lane >> 4)]);
-            if (kSplit) {
-                // tail products first (smaller terms), then the head product
-                bf16x8_t at[4], bt[4];
+            for (int pl = 0; pl < NP; ++pl) {
 #pragma unroll
                 for (int mi = 0; mi < 4; ++mi)
-                    at[mi] = *reinterpret_cast<const bf16x8_t*>(
-                        &As[NP - 1][buf][wm * 64 + mi * 16 + (lane & 15)][8 * (lane >> 4)]);
+                    a[pl][mi] = *reinterpret_cast<const bf16x8_t*>(
+                        &As[pl][buf][wm * 64 + mi * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
                 for (int ni = 0; ni < 4; ++ni)
-                    bt[ni] = *reinterpret_cast<const bf16x8_t*>(
-                        &Bs[NP - 1][buf][wn * 64 + ni * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-                for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < 4; ++ni) {
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[mi], b[ni], acc[mi][ni], 0, 0, 0);
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], bt[ni], acc[mi][ni], 0, 0, 0);
-                    }
+                    b[pl][ni] = *reinterpret_cast<const bf16x8_t*>(
+                        &Bs[pl][buf][wn * 64 + ni * 16 + (lane & 15)][8 * (lane >> 4)]);
             }
+            // plane pairs (pa, pb), smallest products first
 #pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
+            for (int q = 0; q < (NP == 1 ? 1 : (NP == 2 ? 3 : 6)); ++q) {
+                constexpr int kPa3[6] = {2, 0, 1, 1, 0, 0}, kPb3[6] = {0, 2, 1, 0, 1, 0};
+                constexpr int kPa2[3] = {1, 0, 0}, kPb2[3] = {0, 1, 0};
+                const int pa = NP == 3 ? kPa3[q] : (NP == 2 ? kPa2[q] : 0);
+                const int pb = NP == 3 ? kPb3[q] : (NP == 2 ? kPb2[q] : 0);
 #pragma unroll
-                for (int ni = 0; ni < 4; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+                for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 4; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[pa][mi], b[pb][ni], acc[mi][ni], 0, 0, 0);
+            }
             if (s + 1 < nk) store(buf ^ 1);
             __syncthreads();
         }

@@ -53,8 +53,10 @@ __global__ __launch_bounds__(64) void perm_from_pivots_kernel(const long long* _
 // Forward:  Y[b] = L[b][b]^-1 ( B[perm][b] - sum_{j<b} L[b][j] Y[j] )   (unit lower)
 // Backward: X[b] = U[b][b]^-1 ( Y[b]       - sum_{j>b} U[b][j] X[j] )   (B receives X)
 // `nc` right-hand-side columns starting at column c0 of B / Y (row strides ldb / ldy).
-template <typename T, bool kLower>
-__global__ __launch_bounds__(256) void trsv_blocked_kernel(const T* __restrict__ LU, long long lda, long long n,
+// The factors have type TF and are converted to the working type T as they are staged
+// (TF = float, T = double applies low-precision factors in FP64 arithmetic).
+template <typename TF, typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict__ LU, long long lda, long long n,
     const int* __restrict__ perm, const T* __restrict__ Bin, long long ldb_in, T* __restrict__ Out,
     long long ld_out, int nc, unsigned* __restrict__ flags, unsigned spin_limit, int* __restrict__ err,
     const int* __restrict__ skip_if_nonzero)
@@ -73,17 +75,18 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const T* __restrict__
 
     for (int idx = tid; idx < TB * kSolveMaxCols; idx += 256) Acc[idx / kSolveMaxCols][idx % kSolveMaxCols] = T(0);
 
-    // off-diagonal contributions, in increasing distance from the diagonal for the
-    // backward pass and increasing j for the forward pass
+    // off-diagonal contributions in the order the producing blocks complete: increasing j
+    // for the forward pass, decreasing j for the backward pass (block nblk-1 first), so a
+    // workgroup only ever waits for the block finished just before its own
     const long long jcount = kLower ? b : (nblk - 1 - b);
     for (long long q = 0; q < jcount; ++q) {
-        const long long j = kLower ? q : (b + 1 + q);
+        const long long j = kLower ? q : (nblk - 1 - q);
         const long long c0 = j * TB;
         const int cols = static_cast<int>(min(static_cast<long long>(TB), n - c0));
         // stage the L/U block while the producer may still be running
         for (int idx = tid; idx < TB * TB; idx += 256) {
             const int r = idx / TB, c = idx % TB;
-            Ls[r][c] = (r < rows && c < cols) ? LU[(r0 + r) * lda + c0 + c] : T(0);
+            Ls[r][c] = (r < rows && c < cols) ? static_cast<T>(LU[(r0 + r) * lda + c0 + c]) : T(0);
         }
         if (tid == 0) wait_counter(&flags[j], 1u, spin_limit, err);
         __syncthreads();
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const T* __restrict__
     // right-hand side of this block, minus the accumulated contributions
     for (int idx = tid; idx < TB * TB; idx += 256) {
         const int r = idx / TB, c = idx % TB;
-        Ls[r][c] = (r < rows && c < rows) ? LU[(r0 + r) * lda + r0 + c] : T(0);
+        Ls[r][c] = (r < rows && c < rows) ? static_cast<T>(LU[(r0 + r) * lda + r0 + c]) : T(0);
     }
     for (int idx = tid; idx < TB * nc; idx += 256) {
         const int r = idx / nc, c = idx % nc;

@@ -1,0 +1,180 @@
+// linalg_backend::ROCM_NATIVE specialisations of the reference's lu_decomposition and
+// linear_solver templates, as thin C++ wrappers over the C ABI (linalg_rocm_native.h).
+//
+// Reference behaviour kept (paths relative to the reference repository):
+//   argument checks throw       src/matrix_operation/cpu/native/lu_decomposition_native.cpp:30-35,
+//                               src/matrix_operation/cpu/native/linear_solver_native.cpp:24-28
+//   apply returns info == 0     include/matrix_operation/lu_decomposition.h:30-36
+//   workspace_size per type     lu_decomposition_native.cpp:293-326, linear_solver_native.cpp:225-241
+//   Cholesky modes              not part of this backend: they throw std::invalid_argument
+#include <stdexcept>
+#include <string>
+
+#include "../../include/linalg_rocm_native.h"
+#include "../../include/linalg_rocm_native.hpp"
+
+namespace
+{
+
+void check(int code, const char* what)
+{
+    if (code == LRN_OK) return;
+    std::string msg = std::string(what) + ": " + lrn_status_string(code);
+    if (code < 0) throw std::invalid_argument(msg + " (argument " + std::to_string(-code) + ")");
+    throw std::runtime_error(msg);
+}
+
+int solver_type(linalg::linear_solver_type t)
+{
+    const int v = static_cast<int>(t);
+    if (v == LRN_CHOLESKY_LINEAR_SOLVER || v == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER)
+        throw std::invalid_argument("linear_solver<ROCM_NATIVE>: Cholesky modes are not provided by this backend");
+    if (v != LRN_LU_LINEAR_SOLVER && v != LRN_LU_UPFRONT_LINEAR_SOLVER)
+        throw std::invalid_argument("linear_solver<ROCM_NATIVE>: unknown linear_solver_type");
+    return v;
+}
+
+template <typename T> struct ops;
+template <> struct ops<double> {
+    static constexpr auto lu_ws = lrn_lu_decomposition_workspace_size_f64;
+    static constexpr auto lu = lrn_lu_decomposition_f64;
+    static constexpr auto sv_ws = lrn_linear_solver_workspace_size_f64;
+    static constexpr auto sv = lrn_linear_solver_f64;
+    static constexpr auto svm_ws = lrn_linear_solver_multi_workspace_size_f64;
+    static constexpr auto svm = lrn_linear_solver_multi_f64;
+};
+template <> struct ops<float> {
+    static constexpr auto lu_ws = lrn_lu_decomposition_workspace_size_f32;
+    static constexpr auto lu = lrn_lu_decomposition_f32;
+    static constexpr auto sv_ws = lrn_linear_solver_workspace_size_f32;
+    static constexpr auto sv = lrn_linear_solver_f32;
+    static constexpr auto svm_ws = lrn_linear_solver_multi_workspace_size_f32;
+    static constexpr auto svm = lrn_linear_solver_multi_f32;
+};
+
+template <typename T> bool lu_apply(T* m, linalg_int lda, linalg_int* pivot, void* ws, size_t ws_size, int* info, void* stream)
+{
+    if (m == nullptr || pivot == nullptr) throw std::invalid_argument("lu_decomposition<ROCM_NATIVE>: null m or pivot");
+    if (lda <= 0) throw std::invalid_argument("lu_decomposition<ROCM_NATIVE>: lda must be positive");
+    int inf = 0;
+    check(ops<T>::lu(m, lda, reinterpret_cast<int64_t*>(pivot), ws, ws_size, &inf, stream), "lu_decomposition<ROCM_NATIVE>::apply");
+    if (info != nullptr) *info = inf;
+    return inf == 0;
+}
+
+template <typename T> size_t lu_ws(linalg_int lda)
+{
+    size_t b = 0;
+    check(ops<T>::lu_ws(lda, &b), "lu_decomposition<ROCM_NATIVE>::workspace_size");
+    return b;
+}
+
+template <typename T>
+void solve_apply(T* m, linalg_int* pivot, linalg_int lda, T* x, linalg::linear_solver_type type, void* ws,
+    size_t ws_size, void* stream)
+{
+    if (m == nullptr || x == nullptr) throw std::invalid_argument("linear_solver<ROCM_NATIVE>: null m or x");
+    if (lda <= 0) throw std::invalid_argument("linear_solver<ROCM_NATIVE>: lda must be positive");
+    const int t = solver_type(type);
+    check(ops<T>::sv(m, reinterpret_cast<int64_t*>(pivot), lda, x, t, ws, ws_size, stream),
+        "linear_solver<ROCM_NATIVE>::apply");
+}
+
+template <typename T> size_t solve_ws(linalg_int lda, linalg::linear_solver_type type)
+{
+    size_t b = 0;
+    check(ops<T>::sv_ws(lda, solver_type(type), &b), "linear_solver<ROCM_NATIVE>::workspace_size");
+    return b;
+}
+
+}  // namespace
+
+namespace linalg
+{
+
+template <> bool lu_decomposition<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int lda,
+    linalg_int* pivot, void* workspace, size_t workspace_size, int* info, void* stream)
+{
+    return lu_apply<double>(m, lda, pivot, workspace, workspace_size, info, stream);
+}
+template <> bool lu_decomposition<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int lda,
+    linalg_int* pivot, void* workspace, size_t workspace_size, int* info, void* stream)
+{
+    return lu_apply<float>(m, lda, pivot, workspace, workspace_size, info, stream);
+}
+template <> size_t lu_decomposition<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda)
+{
+    return lu_ws<double>(lda);
+}
+template <> size_t lu_decomposition<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda)
+{
+    return lu_ws<float>(lda);
+}
+
+template <> void linear_solver<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int* pivot,
+    linalg_int lda, double* x, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    solve_apply<double>(m, pivot, lda, x, type, workspace, workspace_size, stream);
+}
+template <> void linear_solver<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int* pivot,
+    linalg_int lda, float* x, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    solve_apply<float>(m, pivot, lda, x, type, workspace, workspace_size, stream);
+}
+template <> size_t linear_solver<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return solve_ws<double>(lda, type);
+}
+template <> size_t linear_solver<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return solve_ws<float>(lda, type);
+}
+
+namespace rocm_native
+{
+
+template <typename T> size_t linear_solver_workspace_size(linalg_int lda, linalg_int nrhs, linear_solver_type type)
+{
+    size_t b = 0;
+    check(ops<T>::svm_ws(lda, nrhs, solver_type(type), &b), "rocm_native::linear_solver_workspace_size");
+    return b;
+}
+
+template <typename T>
+void linear_solver_apply(T* m, linalg_int* pivot, linalg_int lda, T* x, linalg_int nrhs, linalg_int ldx,
+    linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    if (m == nullptr || x == nullptr) throw std::invalid_argument("rocm_native::linear_solver_apply: null m or x");
+    check(ops<T>::svm(m, reinterpret_cast<int64_t*>(pivot), lda, x, nrhs, ldx, solver_type(type), workspace,
+              workspace_size, stream),
+        "rocm_native::linear_solver_apply");
+}
+
+template size_t linear_solver_workspace_size<double>(linalg_int, linalg_int, linear_solver_type);
+template size_t linear_solver_workspace_size<float>(linalg_int, linalg_int, linear_solver_type);
+template void linear_solver_apply<double>(double*, linalg_int*, linalg_int, double*, linalg_int, linalg_int,
+    linear_solver_type, void*, size_t, void*);
+template void linear_solver_apply<float>(float*, linalg_int*, linalg_int, float*, linalg_int, linalg_int,
+    linear_solver_type, void*, size_t, void*);
+
+size_t gesv_mixed_workspace_size(linalg_int n, int gmres_restart, bool fallback)
+{
+    size_t b = 0;
+    check(lrn_gesv_mixed_workspace_size_f64(n, gmres_restart, fallback ? 1 : 0, &b),
+        "rocm_native::gesv_mixed_workspace_size");
+    return b;
+}
+
+mixed_result gesv_mixed(const double* a, linalg_int n, const double* b, double* x, void* workspace,
+    size_t workspace_size, double tol, int max_iter, int gmres_restart, void* stream)
+{
+    lrn_mixed_stats st{};
+    check(lrn_gesv_mixed_f64(a, n, b, x, tol, max_iter, gmres_restart, workspace, workspace_size, &st, stream),
+        "rocm_native::gesv_mixed");
+    return mixed_result{st.info, st.path, st.ir_iterations, st.gmres_iterations, st.backward_error};
+}
+
+}  // namespace rocm_native
+}  // namespace linalg

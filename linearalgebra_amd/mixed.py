@@ -41,6 +41,13 @@ def workspace_size(n: int, restart: int = 30, fallback: bool = True) -> int:
     return int(out.value)
 
 
+def set_mixed_update(mode: int) -> None:
+    """Trailing update of the low-precision factorisation (process-wide): 0, 1, 2 = FP32
+    operands split into 1, 2, 3 bf16 planes on the bf16 matrix cores (2 is the default and
+    gives FP32-level factors); 3 = FP32 matrix instructions."""
+    _check(_lib.load().lrn_set_mixed_update(int(mode)), "set_mixed_update")
+
+
 def gesv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, tol: float = 1e-14,
                max_iter: int = 20, restart: int = 30, workspace: torch.Tensor | None = None,
                fallback: bool = True, stream=None):
@@ -63,4 +70,4 @@ def gesv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, 
     return x, st.as_dict()
 
 
-__all__ = ["gesv_mixed", "workspace_size", "MixedStats", "LinalgError"]
+__all__ = ["gesv_mixed", "set_mixed_update", "workspace_size", "MixedStats", "LinalgError"]

@@ -1,0 +1,213 @@
+// C++ test of the linalg_backend::ROCM_NATIVE specialisations (include/linalg_rocm_native.hpp),
+// in the style of the reference's backend tests (TestGpuMatrixOperations.cxx: device buffers,
+// workspace from workspace_size(), throws on invalid arguments, info on singular input).
+//
+//   test_rocm_native_backend --no-gpu   argument checks and workspace sizes only (no HIP call)
+//   test_rocm_native_backend            also LU + solve on the GPU, checked against the CPU
+//                                        oracle (identical pivots, LU and X within 1e-12
+//                                        relative), the zero-pivot KAT and a mixed solve.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "linalg_rocm_native.hpp"
+
+extern "C" {
+int ora_gesv_d(double* m, long long lda, long long* pivot, double* b, long long nrhs, long long ldb);
+}
+
+using linalg::linalg_backend;
+using linalg::linear_solver_type;
+using LU = linalg::lu_decomposition<double, linalg_backend::ROCM_NATIVE>;
+using SV = linalg::linear_solver<double, linalg_backend::ROCM_NATIVE>;
+
+static int g_fail = 0;
+#define EXPECT(cond)                                                         \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            ++g_fail;                                                        \
+        }                                                                    \
+    } while (0)
+
+template <typename E, typename F> static bool throws(F f)
+{
+    try {
+        f();
+    } catch (const E&) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+// SplitMix64 -> uniform (0, 1] -> Box-Muller: a portable N(0, 1) stream
+struct Normal {
+    uint64_t s;
+    bool have = false;
+    double spare = 0.0;
+    explicit Normal(uint64_t seed) : s(seed) {}
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (static_cast<double>(next() >> 11) + 1.0) * 0x1.0p-53; }
+    double operator()()
+    {
+        if (have) {
+            have = false;
+            return spare;
+        }
+        const double r = std::sqrt(-2.0 * std::log(uniform())), t = 2.0 * M_PI * uniform();
+        spare = r * std::sin(t);
+        have = true;
+        return r * std::cos(t);
+    }
+};
+
+static void host_checks()
+{
+    EXPECT(LU::workspace_size(1024) > 0);
+    EXPECT(SV::workspace_size(1024, linear_solver_type::LU_LINEAR_SOLVER) >=
+           SV::workspace_size(1024, linear_solver_type::LU_UPFRONT_LINEAR_SOLVER));
+    long long piv[4];
+    double m[16];
+    double ws[8];
+    EXPECT(throws<std::invalid_argument>([&] { LU::apply(nullptr, 4, piv, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { LU::apply(m, 4, nullptr, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { LU::apply(m, 0, piv, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { LU::apply(m, 4, piv, ws, 1); }));  // workspace too small
+    EXPECT(throws<std::invalid_argument>(
+        [&] { SV::apply(nullptr, piv, 4, m, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { SV::apply(m, piv, 4, m, linear_solver_type::CHOLESKY_LINEAR_SOLVER, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { SV::workspace_size(4, linear_solver_type::CHOLESKY_LINEAR_SOLVER); }));
+}
+
+#define HIP_OK(x) EXPECT((x) == hipSuccess)
+
+static double rel_diff(const std::vector<double>& a, const std::vector<double>& b)
+{
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        num = std::fmax(num, std::fabs(a[i] - b[i]));
+        den = std::fmax(den, std::fabs(b[i]));
+    }
+    return den > 0.0 ? num / den : num;
+}
+
+static void gpu_checks()
+{
+    const long long n = 300;
+    Normal rng(20241203);
+    std::vector<double> a(n * n), x(n);
+    for (auto& v : a) v = rng();
+    for (auto& v : x) v = rng();
+
+    // oracle
+    std::vector<double> lu_ref = a, x_ref = x;
+    std::vector<long long> piv_ref(n);
+    EXPECT(ora_gesv_d(lu_ref.data(), n, piv_ref.data(), x_ref.data(), 1, 1) == 0);
+
+    double *dm = nullptr, *dx = nullptr;
+    long long* dp = nullptr;
+    void* dws = nullptr;
+    const size_t ws = SV::workspace_size(n, linear_solver_type::LU_LINEAR_SOLVER);
+    HIP_OK(hipMalloc(&dm, n * n * sizeof(double)));
+    HIP_OK(hipMalloc(&dx, n * sizeof(double)));
+    HIP_OK(hipMalloc(&dp, n * sizeof(long long)));
+    HIP_OK(hipMalloc(&dws, ws));
+    HIP_OK(hipMemcpy(dm, a.data(), n * n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dx, x.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    SV::apply(dm, dp, n, dx, linear_solver_type::LU_LINEAR_SOLVER, dws, ws);
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<double> lu(n * n), xs(n);
+    std::vector<long long> piv(n);
+    HIP_OK(hipMemcpy(lu.data(), dm, n * n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(xs.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(piv.data(), dp, n * sizeof(long long), hipMemcpyDeviceToHost));
+    EXPECT(piv == piv_ref);
+    const double e_lu = rel_diff(lu, lu_ref), e_x = rel_diff(xs, x_ref);
+    std::printf("n=%lld LU rel diff %.2e, X rel diff %.2e\n", n, e_lu, e_x);
+    EXPECT(e_lu <= 1e-12);
+    EXPECT(e_x <= 1e-12);
+
+    // LU_UPFRONT reuses the factors and pivots
+    HIP_OK(hipMemcpy(dx, x.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    SV::apply(dm, dp, n, dx, linear_solver_type::LU_UPFRONT_LINEAR_SOLVER, dws, ws);
+    HIP_OK(hipMemcpy(xs.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost));
+    EXPECT(rel_diff(xs, x_ref) <= 1e-12);
+
+    // zero-pivot KAT (TestLUDecomposition.cxx:283-299 style): a zero first column -> info 1
+    std::vector<double> z(4 * 4, 0.0);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 1; j < 4; ++j) z[i * 4 + j] = 1.0 + i + 2.0 * j;
+    HIP_OK(hipMemcpy(dm, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    int info = -1;
+    const bool ok = LU::apply(dm, 4, dp, dws, LU::workspace_size(4), &info);
+    EXPECT(!ok);
+    EXPECT(info == 1);
+
+    // mixed-precision solve of a diagonally dominant system
+    const long long nm = 512;
+    std::vector<double> am(nm * nm), bm(nm);
+    for (long long i = 0; i < nm; ++i) {
+        for (long long j = 0; j < nm; ++j) am[i * nm + j] = rng();
+        am[i * nm + i] += static_cast<double>(nm);
+        bm[i] = rng();
+    }
+    double *da = nullptr, *db = nullptr, *dxm = nullptr;
+    void* dwm = nullptr;
+    const size_t wsm = linalg::rocm_native::gesv_mixed_workspace_size(nm);
+    HIP_OK(hipMalloc(&da, nm * nm * sizeof(double)));
+    HIP_OK(hipMalloc(&db, nm * sizeof(double)));
+    HIP_OK(hipMalloc(&dxm, nm * sizeof(double)));
+    HIP_OK(hipMalloc(&dwm, wsm));
+    HIP_OK(hipMemcpy(da, am.data(), am.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(db, bm.data(), bm.size() * sizeof(double), hipMemcpyHostToDevice));
+    const auto r = linalg::rocm_native::gesv_mixed(da, nm, db, dxm, dwm, wsm);
+    std::vector<double> xm(nm);
+    HIP_OK(hipMemcpy(xm.data(), dxm, nm * sizeof(double), hipMemcpyDeviceToHost));
+    double rn = 0.0, an = 0.0, xn = 0.0, bn = 0.0;
+    for (long long i = 0; i < nm; ++i) {
+        double s = bm[i], rs = 0.0;
+        for (long long j = 0; j < nm; ++j) {
+            s -= am[i * nm + j] * xm[j];
+            rs += std::fabs(am[i * nm + j]);
+        }
+        rn = std::fmax(rn, std::fabs(s));
+        an = std::fmax(an, rs);
+        xn = std::fmax(xn, std::fabs(xm[i]));
+        bn = std::fmax(bn, std::fabs(bm[i]));
+    }
+    const double berr = rn / (an * xn + bn);
+    std::printf("mixed n=%lld path=%d ir=%d backward error %.2e\n", nm, r.path, r.ir_iterations, berr);
+    EXPECT(berr <= 1e-14);
+    EXPECT(r.path == 0);
+    (void)hipFree(dm);
+    (void)hipFree(dx);
+    (void)hipFree(dp);
+    (void)hipFree(dws);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dxm);
+    (void)hipFree(dwm);
+}
+
+int main(int argc, char** argv)
+{
+    const bool no_gpu = argc > 1 && std::strcmp(argv[1], "--no-gpu") == 0;
+    host_checks();
+    if (!no_gpu) gpu_checks();
+    if (g_fail == 0) std::printf("all checks passed (%s)\n", no_gpu ? "host only" : "host + GPU");
+    return g_fail == 0 ? 0 : 1;
+}
