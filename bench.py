@@ -34,9 +34,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 METRIC = "FP64-accurate Ax=b solves/s: mixed LU-IR vs plain FP64 LU, n=16384; batch solves/s"
 PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 TRACE = {"gemm_f64": 0, "gemm_f64_edge": 1, "gemm_bf16x3": 2, "gemm_bf16": 3, "gemm_f32": 4, "panel": 5,
-         "laswp": 6, "trsm": 7, "trsv": 8}
-FLOP_KINDS = {"gemm_f64", "gemm_f64_edge", "gemm_bf16x3", "gemm_bf16", "gemm_f32", "panel"}
-KERNEL_SYMBOL = {"gemm_f64": "gemm_sub_f64_kernel<false>", "gemm_bf16x3": "gemm_sub_bf16_kernel<true>"}
+         "laswp": 6, "trsm": 7, "trsv": 8, "gemm_bf16x6": 9}
+FLOP_KINDS = {"gemm_f64", "gemm_f64_edge", "gemm_bf16x3", "gemm_bf16", "gemm_f32", "panel", "gemm_bf16x6"}
+KERNEL_SYMBOL = {"gemm_f64": "gemm_sub_f64_kernel<false>", "gemm_bf16x3": "gemm_sub_bf16_kernel<2>",
+                 "gemm_bf16x6": "gemm_sub_bf16_kernel<3>", "gemm_bf16": "gemm_sub_bf16_kernel<1>"}
+# bf16 matrix-core products per algorithmic product, by trailing-update mode
+PLANE_PRODUCTS = {"gemm_bf16": 1, "gemm_bf16x3": 3, "gemm_bf16x6": 6}
 
 
 def log(msg: str) -> None:
@@ -56,6 +59,8 @@ def parse():
     p.add_argument("--cpu-n", type=int, default=2048, help="order of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--mixed-update", type=int, default=3,
+                   help="trailing update of the mixed factorisation: 0/1/2 = 1/2/3 bf16 planes, 3 = FP32 MFMA")
     return p.parse_args()
 
 
@@ -94,10 +99,15 @@ def roofline(lib, kind, peak_tflops, traffic):
     if c == 0 or ms <= 0:
         return None
     achieved = w / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "kernel": KERNEL_SYMBOL.get(kind, kind), "achieved": round(achieved, 2),
-            "peak": peak_tflops, "unit": "TFLOP/s", "frac": round(achieved / peak_tflops, 4), "traffic": traffic,
-            "launches": c, "avg_launch_us": round(1e3 * ms / c, 2),
-            "flops_per_launch": round(w / c, 1)}
+    out = {"bound": "mfma", "kernel": KERNEL_SYMBOL.get(kind, kind), "achieved": round(achieved, 2),
+           "peak": peak_tflops, "unit": "TFLOP/s", "frac": round(achieved / peak_tflops, 4), "traffic": traffic,
+           "launches": c, "avg_launch_us": round(1e3 * ms / c, 2),
+           "flops_per_launch": round(w / c, 1), "flops": "algorithmic 2*M*N*K per launch"}
+    if kind in PLANE_PRODUCTS:
+        k = PLANE_PRODUCTS[kind]
+        out["matrix_core_flops_per_launch"] = round(k * w / c, 1)
+        out["matrix_core_frac"] = round(k * achieved / peak_tflops, 4)
+    return out
 
 
 def traffic_from_profiles(kind):
@@ -205,13 +215,15 @@ def main():
     log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
 
     # ---------------- mixed LU-IR (headline)
+    lam.set_mixed_update(args.mixed_update)
+    gemm_kind = {0: "gemm_bf16", 1: "gemm_bf16x3", 2: "gemm_bf16x6", 3: "gemm_f32"}[args.mixed_update]
     ws = torch.empty(lam.workspace_size(n, args.restart, True), dtype=torch.uint8, device=dev)
     x = torch.empty(n, dtype=torch.float64, device=dev)
     for i in range(warmup):
         _, st = la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=args.restart, workspace=ws)
         log(f"mixed warmup {i}: path={st['path']} ir={st['ir_iterations']} gmres={st['gmres_iterations']} "
             f"berr={st['backward_error']:.2e} factor={st['factor_ms']:.1f} ms refine={st['refine_ms']:.1f} ms")
-    trace_start(lib, ["gemm_bf16x3"])
+    trace_start(lib, [gemm_kind])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
@@ -221,7 +233,8 @@ def main():
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     lib.lrn_trace_stop()
-    roof_mixed = roofline(lib, "gemm_bf16x3", PEAK["bf16_mfma_tflops"], traffic_from_profiles("gemm_bf16x3"))
+    roof_mixed = roofline(lib, gemm_kind, PEAK["fp32_tflops"] if gemm_kind == "gemm_f32" else PEAK["bf16_mfma_tflops"],
+                          traffic_from_profiles(gemm_kind))
     log(f"mixed: {1e3 * t_mixed / steps:.1f} ms/solve")
     berr = backward_error_gpu(torch, a, x, b)
     trace_start(lib, list(TRACE))
@@ -296,7 +309,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 panels + split-bf16 MFMA update (fp32 accumulate), f64 residual/refinement",
+        "dtype": {0: "f32 panels + bf16 MFMA update", 1: "f32 panels + 2-plane bf16 MFMA update",
+                  2: "f32 panels + 3-plane bf16 MFMA update", 3: "f32 panels + f32 MFMA update"}[args.mixed_update]
+                 + " (fp32 accumulate); f64 residual and refinement",
         "data": "synthetic",
         "config": {"workload": f"mixed-precision LU-IR solve of A x = b, n={n}, nrhs=1, "
                                f"A = U diag(s) V^T (Haar U, V; s log-spaced 1..1/kappa), kappa={args.kappa:g}, "

@@ -123,8 +123,9 @@ typedef struct lrn_mixed_stats {
 /* Trailing update of the low-precision factorisation (process-wide): every FP32 operand is
  * split into bf16 planes x ~ p0 + p1 + p2 and the plane products run on the bf16 matrix
  * cores with FP32 accumulation. mode 0 = one plane (bf16 operands), 1 = two planes (three
- * products, ~2^-16 relative), 2 = three planes (six products, FP32-level factors; the
- * default), 3 = FP32 matrix instructions. */
+ * products, ~2^-16 relative), 2 = three planes (six products, FP32-level factors),
+ * 3 = FP32 matrix instructions (the default). Modes 2 and 3 give FP32-level factors, which
+ * LU-IR needs at kappa ~ 1e6; modes 0 and 1 rely on GMRES-IR there. */
 LRN_API int lrn_set_mixed_update(int mode);
 LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int fallback, size_t* bytes);
 LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,

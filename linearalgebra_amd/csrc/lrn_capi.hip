@@ -113,6 +113,7 @@ template <typename T> struct GetrsLayout {
     unsigned* flags_b;
     int* err;
     T* y;
+    Chunk* xch;  // single right-hand side: tagged exchange of solved values, one per row
 };
 
 long long solve_blocks(long long n) { return (n + kSolveTB - 1) / kSolveTB; }
@@ -126,6 +127,7 @@ template <typename T> GetrsLayout<T> carve_getrs(Arena& ar, long long n, long lo
     L.flags_b = ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
     L.err = ar.next<int>(64);
     L.y = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(std::min<long long>(nrhs, kSolveMaxCols)));
+    L.xch = ar.next<Chunk>(static_cast<size_t>(n));
     return L;
 }
 
@@ -392,8 +394,9 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
 }
 
 // Trailing update of the low-precision factorisation: 0 = bf16 operands, 1 = two bf16 planes
-// (three products), 2 = three bf16 planes (six products, FP32-level), 3 = FP32 MFMA.
-std::atomic<int> g_mixed_update{2};
+// (three products), 2 = three bf16 planes (six products, FP32-level), 3 = FP32 MFMA (the
+// default: FP32-level factors, and on gfx950 faster than the six-product form of this build).
+std::atomic<int> g_mixed_update{3};
 
 // C[M x N] -= A * B, FP32 data, with `planes` bf16 planes per operand (1..3)
 int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long long ldb, float* C, long long ldc,
@@ -669,6 +672,20 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
     const long long nblk = solve_blocks(n);
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
+        if (nc == 1) {
+            // latency-optimised single right-hand side: tags 1 (forward) and 2 (backward)
+            LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.xch), 0u, 4 * n, s));
+            LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
+            const double tri_bytes = (0.5 * static_cast<double>(n) * n) * sizeof(TF) + 4.0 * n * sizeof(T);
+            TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
+            hipLaunchKernelGGL((trsv1_kernel<TF, T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n,
+                n, L.perm, B + c0, ldb, L.y, 1ll, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+            LRN_HIP(hipGetLastError());
+            hipLaunchKernelGGL((trsv1_kernel<TF, T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n,
+                n, L.perm, L.y, 1ll, B + c0, ldb, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+            LRN_HIP(hipGetLastError());
+            continue;
+        }
         LRN_TRY(launch_fill(L.flags_f, 0u, nblk, s));
         LRN_TRY(launch_fill(L.flags_b, 0u, nblk, s));
         LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));

@@ -117,7 +117,7 @@ def test_getrf_singular_larger(dev):
     assert O.rel_diff(lu_g, lu_o) <= 1e-12
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (65, 1), (300, 3), (1024, 16), (1500, 64), (777, 70)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (65, 1), (300, 3), (300, 65), (1024, 16), (1500, 64), (777, 70), (4100, 1)])
 def test_solver_lu_mode_matches_oracle(n, nrhs, dev):
     a = O.gaussian(n, seed=11 + n)
     b = O.gaussian(n, nrhs, seed=12 + n)

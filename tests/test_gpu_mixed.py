@@ -97,7 +97,7 @@ def test_gesv_mixed_update_modes(mode, dev):
         x, st = la.gesv_mixed(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev), tol=1e-14, max_iter=30,
                               restart=40)
     finally:
-        lam.set_mixed_update(2)
+        lam.set_mixed_update(3)
     berr = O.backward_error(a, x.cpu().numpy(), b)
     print(f"mode={mode} path={st['path_name']} ir={st['ir_iterations']} gmres={st['gmres_iterations']} "
           f"berr={berr:.2e}", flush=True)
