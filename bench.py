@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--restart", type=int, default=30)
     p.add_argument("--cpu-n", type=int, default=2048, help="order of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--matrix", choices=["conditioned", "gaussian"], default="conditioned",
+                   help="conditioned: U diag(s) V^T with the given kappa (the benchmark input); gaussian: iid "
+                        "N(0,1) entries, no QR (for profiler runs that cannot host the QR)")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--mixed-update", type=int, default=3,
                    help="trailing update of the mixed factorisation: 0/1/2 = 1/2/3 bf16 planes, 3 = FP32 MFMA")
@@ -208,9 +211,15 @@ def main():
     lib = _lib.load()
     dev = torch.device("cuda:0")
     n, steps, warmup = args.n, args.steps, args.warmup
-    log(f"generating A (n={n}, kappa={args.kappa:g}) on the GPU")
+    log(f"generating A (n={n}, {args.matrix}, kappa={args.kappa:g}) on the GPU")
     t0 = time.perf_counter()
-    a, b = conditioned_gpu(torch, n, args.kappa, args.seed, dev)
+    if args.matrix == "conditioned":
+        a, b = conditioned_gpu(torch, n, args.kappa, args.seed, dev)
+    else:
+        g = torch.Generator(device=dev)
+        g.manual_seed(args.seed)
+        a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        b = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
     torch.cuda.synchronize()
     log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
 
@@ -316,7 +325,8 @@ def main():
         "config": {"workload": f"mixed-precision LU-IR solve of A x = b, n={n}, nrhs=1, "
                                f"A = U diag(s) V^T (Haar U, V; s log-spaced 1..1/kappa), kappa={args.kappa:g}, "
                                f"b iid N(0,1), backward error <= 1e-14",
-                   "n": n, "nrhs": 1, "kappa": args.kappa, "seed": args.seed, "parallelism": "single GPU"},
+                   "n": n, "nrhs": 1, "kappa": args.kappa if args.matrix == "conditioned" else None,
+                   "matrix": args.matrix, "seed": args.seed, "parallelism": "single GPU"},
         "backward_error": berr,
         "library_backward_error": last["backward_error"],
         "path": ["lu-ir", "gmres-ir", "fp64-fallback", "not-converged"][last["path"]],
