@@ -62,6 +62,9 @@ def parse():
                    help="conditioned: U diag(s) V^T with the given kappa (the benchmark input); gaussian: iid "
                         "N(0,1) entries, no QR (for profiler runs that cannot host the QR)")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--batch-count", type=int, default=10000)
+    p.add_argument("--batch-replays", type=int, default=20)
     p.add_argument("--mixed-update", type=int, default=3,
                    help="trailing update of the mixed factorisation: 0/1/2 = 1/2/3 bf16 planes, 3 = FP32 MFMA")
     return p.parse_args()
@@ -156,6 +159,65 @@ def backward_error_gpu(torch, a, x, b):
     r = b - a @ x
     den = torch.linalg.matrix_norm(a, ord=float("inf")) * x.abs().max() + b.abs().max()
     return float(r.abs().max() / den)
+
+
+# ----------------------------------------------------------------------------- batched
+
+def batched_bench(torch, count, replays, seed, dev):
+    """The batched workload: `count` FP64 (and FP32) systems, n ~ U[8, 256], nrhs ~ U[1, 8],
+    iid N(0,1) entries, solved by the ordering + persistent kernels captured in one graph.
+    Median graph-replay time over `replays` (A and B restored eagerly, untimed, in between)."""
+    import numpy as np
+    from linearalgebra_amd.batched import BatchedSystems
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = rng.integers(8, 257, size=count)
+    r = rng.integers(1, 9, size=count)
+    nf = n.astype(np.float64)
+    flops = float((2.0 / 3.0 * nf ** 3 + 2.0 * nf ** 2 * r).sum())
+    out = {}
+    for dt, name, peak in ((torch.float64, "f64", PEAK["fp64_mfma_tflops"]), (torch.float32, "f32", PEAK["fp32_tflops"])):
+        s = BatchedSystems(n, r, dtype=dt, device=dev)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        s.a.copy_(torch.randn(s.a.numel(), dtype=dt, device=dev, generator=g))
+        s.b.copy_(torch.randn(s.b.numel(), dtype=dt, device=dev, generator=g))
+        a0, b0 = s.a.clone(), s.b.clone()
+        s.solve()  # eager first call (kernel set-up outside capture)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        s.a.copy_(a0)
+        s.b.copy_(b0)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            s.solve()
+        times = []
+        for _ in range(replays):
+            s.a.copy_(a0)
+            s.b.copy_(b0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            graph.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        ms = float(np.median(times))
+        ok = bool((s.info.cpu() == 0).all())
+        elt = 8 if dt == torch.float64 else 4
+        bytes_alg = float(((2.0 * nf ** 2 + 2.0 * nf * r) * elt).sum())
+        tf = flops / (ms * 1e-3) / 1e12
+        out[name] = {"ms": round(ms, 3), "solves_per_s": round(count / (ms * 1e-3), 1), "tflops": round(tf, 3),
+                     "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s",
+                                  "frac": round(tf / peak, 4),
+                                  "hbm_gbs": round(bytes_alg / (ms * 1e-3) / 1e9, 1), "hbm_frac":
+                                      round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK["hbm_gbs"], 4)},
+                     "all_info_zero": ok, "replays": replays}
+        log(f"batched {name}: {ms:.3f} ms median ({tf:.2f} TF/s)")
+        del s, a0, b0, graph
+    return {"workload": f"{count} independent systems, n ~ U[8,256], nrhs ~ U[1,8], iid N(0,1); ordering + "
+                        f"persistent kernel captured in one graph, median of {replays} replays",
+            "flops": flops, **out}
 
 
 # ----------------------------------------------------------------------------- CPU baseline
@@ -300,6 +362,10 @@ def main():
                  "lu_tflops": round(2.0 * n ** 3 / 3.0 / (t_plain / steps) / 1e12, 2)}
         del aw, wsp
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(torch, args.batch_count, args.batch_replays, args.seed, dev)
+
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline at n={args.cpu_n}")
@@ -341,6 +407,7 @@ def main():
         "roofline_fp64_lu": roof_fp64,
         "phases_mixed": phases_mixed,
         "phases_plain_fp64": phases_plain,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     if not (berr <= 1e-14):

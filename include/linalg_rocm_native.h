@@ -131,6 +131,29 @@ LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int 
 LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,
     int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream);
 
+/* ---- batched solve of many small systems (no reference counterpart) --------------------
+ * `count` independent systems A_i X_i = B_i solved in one persistent kernel (plus a small
+ * ordering kernel; both are stream-ordered launches with no memset and no host sync, so
+ * the call can be captured in a hipGraph once the first call has run outside capture).
+ * System i: order n[i] in [1, 256], nrhs[i] in [0, 8] right-hand sides; A_i row-major
+ * n_i x n_i at a + a_offset[i] (elements), factored in place (packed L\U as
+ * lu_decomposition); pivots (1-based, as lu_decomposition) at pivot + pivot_offset[i];
+ * B_i row-major n_i x nrhs_i at b + b_offset[i], overwritten with X_i only when the
+ * factorisation is nonsingular (linear_solver LU mode); info[i] = 0, the first zero pivot
+ * (1-based), or -1 for an n / nrhs outside the supported range (system skipped).
+ * Every array is device memory. Asynchronous. Pivot rule, elimination and solve order
+ * follow lu_decomposition / linear_solver. */
+LRN_API int lrn_batched_gesv_workspace_size(int64_t count, size_t* bytes);
+LRN_API int lrn_batched_gesv_f64(double* a, const int64_t* a_offset, int64_t* pivot, const int64_t* pivot_offset,
+    double* b, const int64_t* b_offset, const int32_t* n, const int32_t* nrhs, int64_t count, int32_t* info,
+    void* workspace, size_t workspace_size, void* stream);
+LRN_API int lrn_batched_gesv_f32(float* a, const int64_t* a_offset, int64_t* pivot, const int64_t* pivot_offset,
+    float* b, const int64_t* b_offset, const int32_t* n, const int32_t* nrhs, int64_t count, int32_t* info,
+    void* workspace, size_t workspace_size, void* stream);
+/* Workgroups of the persistent batched launch (0 = occupancy x compute units, the default);
+ * used by the autotuner. Process-wide. */
+LRN_API int lrn_set_batched_grid(int workgroups);
+
 /* Device tuning knobs (outer block, panel width, panel workgroups) for the autotuner. A
  * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);
@@ -172,7 +195,8 @@ enum lrn_trace_kind {
     LRN_TRACE_LASWP = 6,           /* row interchanges (bytes, every pivot taken as a swap) */
     LRN_TRACE_TRSM = 7,            /* U12 triangular solve (bytes) */
     LRN_TRACE_TRSV = 8,            /* solve substitutions (bytes) */
-    LRN_TRACE_GEMM_BF16X6 = 9      /* three-plane bf16 trailing GEMM (FP32-equivalent flops 2MNK) */
+    LRN_TRACE_GEMM_BF16X6 = 9,     /* three-plane bf16 trailing GEMM (FP32-equivalent flops 2MNK) */
+    LRN_TRACE_BATCHED = 10         /* batched solve kernel (work 0: not known on the host) */
 };
 LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity);
 LRN_API int lrn_trace_stop(void);

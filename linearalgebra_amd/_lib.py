@@ -39,6 +39,10 @@ SIGNATURES = {
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
     "lrn_set_mixed_update": [_int],
+    "lrn_batched_gesv_workspace_size": [_i64, ctypes.POINTER(_sz)],
+    "lrn_batched_gesv_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp],
+    "lrn_batched_gesv_f32": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp],
+    "lrn_set_batched_grid": [_int],
     "lrn_trace_start": [ctypes.c_uint, _i64],
     "lrn_trace_stop": [],
     "lrn_trace_read": [_int, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],

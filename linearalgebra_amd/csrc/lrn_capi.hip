@@ -33,6 +33,7 @@
 #include "lrn_gemm_bf16.hpp"
 #include "lrn_mixed.hpp"
 #include "lrn_trace.hpp"
+#include "lrn_batched.hpp"
 
 namespace lrn {
 namespace {
@@ -1108,6 +1109,58 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     return LRN_OK;
 }
 
+// ------------------------------------------------------------------- batched solve
+
+struct BatchLayout {
+    int* order;
+    unsigned* ctr;
+};
+
+BatchLayout carve_batch(Arena& ar, long long count)
+{
+    BatchLayout L;
+    L.order = ar.next<int>(static_cast<size_t>(std::max<long long>(count, 1)));
+    L.ctr = ar.next<unsigned>(64);
+    return L;
+}
+
+size_t batch_ws_bytes(long long count)
+{
+    Arena ar;
+    carve_batch(ar, count);
+    return ar.off;
+}
+
+std::atomic<int> g_batch_grid{0};  // workgroups of the persistent launch (0 = occupancy x CUs)
+
+template <typename T>
+int batched_gesv_impl(T* a, const int64_t* a_off, int64_t* piv, const int64_t* piv_off, T* b, const int64_t* b_off,
+    const int32_t* n, const int32_t* nrhs, int64_t count, int32_t* info, void* ws, size_t ws_size, hipStream_t s)
+{
+    if (count == 0) return LRN_OK;
+    Arena ar(ws, ws_size);
+    const BatchLayout L = carve_batch(ar, count);
+    hipLaunchKernelGGL(batch_order_kernel, dim3(1), dim3(1024), 0, s, n, nrhs, static_cast<int>(count), L.order, L.ctr,
+        info);
+    LRN_HIP(hipGetLastError());
+    constexpr int NB = 16;
+    auto kern = batched_gesv_kernel<T, NB>;
+    int per_cu = 0;
+    LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), 0, kBatThreads, &per_cu));
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    long long grid = static_cast<long long>(std::max(per_cu, 1)) * device_props(dev).cus;
+    if (g_batch_grid.load() > 0) grid = g_batch_grid.load();
+    grid = std::min<long long>(grid, count);
+    TraceScope ts(kTrBatched, 0.0, s);
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(kBatThreads), 0, s, a,
+        reinterpret_cast<const long long*>(a_off), reinterpret_cast<long long*>(piv),
+        reinterpret_cast<const long long*>(piv_off), b, reinterpret_cast<const long long*>(b_off), n, nrhs, info,
+        L.order, L.ctr, static_cast<int>(count));
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
 }  // namespace
 }  // namespace lrn
 
@@ -1277,6 +1330,52 @@ LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, doub
     if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
     return gesv_mixed_impl(a, n, b, x, tol, max_iter, gmres_restart, workspace, workspace_size, stats,
         static_cast<hipStream_t>(stream));
+}
+
+LRN_API int lrn_batched_gesv_workspace_size(int64_t count, size_t* bytes)
+{
+    if (count < 0 || count > (1ll << 30)) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = batch_ws_bytes(count);
+    return LRN_OK;
+}
+
+#define LRN_BATCHED_ARGS_CHECK()                                                                   \
+    do {                                                                                           \
+        if (count < 0 || count > (1ll << 30)) return -9;                                           \
+        if (count == 0) return LRN_OK;                                                             \
+        if (a == nullptr || a_offset == nullptr) return -1;                                        \
+        if (pivot == nullptr || pivot_offset == nullptr) return -3;                                \
+        if (b == nullptr || b_offset == nullptr) return -5;                                        \
+        if (n == nullptr) return -7;                                                               \
+        if (nrhs == nullptr) return -8;                                                            \
+        if (info == nullptr) return -10;                                                           \
+        if (workspace == nullptr || workspace_size < batch_ws_bytes(count)) return -11;            \
+    } while (0)
+
+LRN_API int lrn_batched_gesv_f64(double* a, const int64_t* a_offset, int64_t* pivot, const int64_t* pivot_offset,
+    double* b, const int64_t* b_offset, const int32_t* n, const int32_t* nrhs, int64_t count, int32_t* info,
+    void* workspace, size_t workspace_size, void* stream)
+{
+    LRN_BATCHED_ARGS_CHECK();
+    return batched_gesv_impl<double>(a, a_offset, pivot, pivot_offset, b, b_offset, n, nrhs, count, info, workspace,
+        workspace_size, static_cast<hipStream_t>(stream));
+}
+
+LRN_API int lrn_batched_gesv_f32(float* a, const int64_t* a_offset, int64_t* pivot, const int64_t* pivot_offset,
+    float* b, const int64_t* b_offset, const int32_t* n, const int32_t* nrhs, int64_t count, int32_t* info,
+    void* workspace, size_t workspace_size, void* stream)
+{
+    LRN_BATCHED_ARGS_CHECK();
+    return batched_gesv_impl<float>(a, a_offset, pivot, pivot_offset, b, b_offset, n, nrhs, count, info, workspace,
+        workspace_size, static_cast<hipStream_t>(stream));
+}
+
+LRN_API int lrn_set_batched_grid(int workgroups)
+{
+    if (workgroups < 0) return -1;
+    g_batch_grid.store(workgroups);
+    return LRN_OK;
 }
 
 LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity)

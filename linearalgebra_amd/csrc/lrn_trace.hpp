@@ -25,7 +25,8 @@ enum TraceKind {
     kTrTrsm = 7,         // unit-lower TRSM of U12 rows, bytes
     kTrTrsv = 8,         // forward / backward substitution of the solve, bytes
     kTrGemmBf16x6 = 9,   // gemm_sub_bf16_kernel<3>  (three planes), FP32-equivalent flops 2MNK
-    kTrKinds = 10
+    kTrBatched = 10,     // batched_gesv_kernel (work not known on the host: 0)
+    kTrKinds = 11
 };
 
 struct TraceRec {
