@@ -5,8 +5,8 @@
 // resets the work counter, so it can sit in a captured graph without a memset node.
 // batched_gesv_kernel: every workgroup repeatedly takes the next system from the sorted
 // list (one atomic per system) and solves it alone:
-//   factorisation  right-looking, blocked by NB columns: the NB-column panel is factored in
-//                  LDS with partial pivoting (the reference's pivot rule: first maximum of
+//   factorisation  right-looking, blocked by NB columns: the NB-column panel is factored
+//                  with one panel row per thread, in registers, with partial pivoting (the reference's pivot rule: first maximum of
 //                  |a| wins, NaN never chosen, exactly-zero pivots skipped and reported,
 //                  reciprocal multiplier unless |d| < T_MIN), its interchanges are applied
 //                  to the other columns, U12 = L11^-1 A12 (row order of the reference's
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(1024) void batch_order_kernel(const int* __restrict
 }
 
 template <typename T, int NB>
-__global__ __launch_bounds__(kBatThreads) void batched_gesv_kernel(T* __restrict__ a, const long long* __restrict__ a_off,
+__global__ __launch_bounds__(kBatThreads, 3) void batched_gesv_kernel(T* __restrict__ a, const long long* __restrict__ a_off,
     long long* __restrict__ piv, const long long* __restrict__ piv_off, T* __restrict__ b,
     const long long* __restrict__ b_off, const int* __restrict__ nv, const int* __restrict__ nrhsv,
     int* __restrict__ info, const int* __restrict__ order, unsigned* __restrict__ ctr, int count)
@@ -74,13 +74,21 @@ __global__ __launch_bounds__(kBatThreads) void batched_gesv_kernel(T* __restrict
     constexpr int UC = 64;
     __shared__ T P[kBatMaxN][NB + 1];
     __shared__ T Us[NB][UC + 1];
-    __shared__ T Bs[kBatMaxN][kBatMaxRhs + 1];
     __shared__ int perm[kBatMaxN];
     __shared__ double red_v[kBatThreads / kWave];
     __shared__ long long red_i[kBatThreads / kWave];
-    __shared__ int sw[NB];
+    __shared__ long long pivs[NB];
+    __shared__ T prow[NB];
+    __shared__ T jrow[NB];
+    __shared__ T prcp;
+    __shared__ int rowmap[1 + 4 * NB];
     __shared__ int s_sys;
     __shared__ int s_info;
+
+    // the right-hand sides of the solve reuse the panel's LDS (the solve starts after the
+    // factorisation's last barrier)
+    static_assert(kBatMaxRhs + 1 <= NB + 1, "B block must fit in the panel buffer");
+    T (*const Bs)[kBatMaxRhs + 1] = reinterpret_cast<T (*)[kBatMaxRhs + 1]>(&P[0][0]);
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -108,96 +116,134 @@ __global__ __launch_bounds__(kBatThreads) void batched_gesv_kernel(T* __restrict
         for (int k = 0; k < n; k += NB) {
             const int w = min(NB, n - k);
             const int m = n - k;
-            for (int idx = tid; idx < m * w; idx += kBatThreads) {
-                const int r = idx / w, c = idx - (idx / w) * w;
-                P[r][c] = A[static_cast<long long>(k + r) * n + k + c];
+            // panel: thread t holds panel row t (m <= 256 = threads) in registers; per column
+            // a DPP wave argmax, one barrier to combine the four waves, one to publish the
+            // pivot row and the diagonal row, then the interchange and the rank-1 update in
+            // registers
+            T rowv[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                rowv[c] = (tid < m && c < w) ? A[static_cast<long long>(k + tid) * n + k + c] : T(0);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j < w) {
+                    double key = (tid >= j && tid < m) ? pivot_key(rowv[j], tid == j) : -2.0;
+                    long long ri = tid;
+                    wave_argmax_ordered(key, ri);
+                    if (lane == 0) {
+                        red_v[wid] = key;
+                        red_i[wid] = ri;
+                    }
+                    __syncthreads();
+                    double bv = red_v[0];
+                    long long bi = red_i[0];
+#pragma unroll
+                    for (int qq = 1; qq < kBatThreads / kWave; ++qq)
+                        if (argmax_better(red_v[qq], red_i[qq], bv, bi)) {
+                            bv = red_v[qq];
+                            bi = red_i[qq];
+                        }
+                    const int p = static_cast<int>(bi);
+                    if (tid == p) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) prow[c] = rowv[c];
+                        // the multiplier's reciprocal, computed once (|d| >= T_MIN: x * (1/d))
+                        const T d0 = rowv[j];
+                        prcp = (d0 != T(0) && fabs(static_cast<double>(d0)) >= static_cast<double>(TypeMin<T>::v))
+                                   ? T(1) / d0
+                                   : T(0);
+                    }
+                    if (tid == j) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) jrow[c] = rowv[c];
+                    }
+                    if (tid == 0) {
+                        pv[k + j] = static_cast<long long>(k + p + 1);
+                        pivs[j] = static_cast<long long>(k + p);
+                        if (p != j) {
+                            const int t = perm[k + j];
+                            perm[k + j] = perm[k + p];
+                            perm[k + p] = t;
+                        }
+                    }
+                    __syncthreads();
+                    if (p != j) {
+                        if (tid == j) {
+#pragma unroll
+                            for (int c = 0; c < NB; ++c) rowv[c] = prow[c];
+                        } else if (tid == p) {
+#pragma unroll
+                            for (int c = 0; c < NB; ++c) rowv[c] = jrow[c];
+                        }
+                    }
+                    const T d = prow[j];
+                    if (d == T(0)) {
+                        if (tid == 0 && s_info == 0) s_info = k + j + 1;
+                    } else if (tid > j && tid < m) {
+                        const T rcp = prcp;
+                        const T l = (rcp != T(0)) ? rowv[j] * rcp : rowv[j] / d;
+                        rowv[j] = l;
+#pragma unroll
+                        for (int c = j + 1; c < NB; ++c)
+                            if (c < w) rowv[c] = rowv[c] - l * prow[c];
+                    }
+                }
+            }
+            if (tid < m) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < w) {
+                        P[tid][c] = rowv[c];
+                        A[static_cast<long long>(k + tid) * n + k + c] = rowv[c];
+                    }
             }
             __syncthreads();
-            for (int j = 0; j < w; ++j) {
-                double key = -2.0;
-                long long ri = 0x7fffffffffffffffll;
-                if (tid >= j && tid < m) {
-                    key = pivot_key(P[tid][j], tid == j);
-                    ri = tid;
-                }
-                wave_argmax(key, ri);
-                if (lane == 0) {
-                    red_v[wid] = key;
-                    red_i[wid] = ri;
-                }
-                __syncthreads();
-                double bv = red_v[0];
-                long long bi = red_i[0];
+            if (wid == 0) panel_rowmap(pivs, k, w, 2 * NB, rowmap);
+            __syncthreads();
+            // the panel's interchanges on every other column (one column per thread): the
+            // net row moves (rowmap: row dst receives row src) in one gather and one scatter
+            {
+                const int cnt = rowmap[0];
+                const int* dst = rowmap + 1;
+                const int* src = rowmap + 1 + 2 * NB;
+                const int c = tid;  // n <= 256 = threads
+                if (c < n && !(c >= k && c < k + w)) {
+                    // all reads before any write (the moves form a permutation of the rows)
+                    T v[2 * NB];
 #pragma unroll
-                for (int qq = 1; qq < kBatThreads / kWave; ++qq)
-                    if (argmax_better(red_v[qq], red_i[qq], bv, bi)) {
-                        bv = red_v[qq];
-                        bi = red_i[qq];
-                    }
-                const int p = static_cast<int>(bi);
-                if (p != j && tid < w) {
-                    const T t = P[j][tid];
-                    P[j][tid] = P[p][tid];
-                    P[p][tid] = t;
-                }
-                if (tid == 0) {
-                    pv[k + j] = static_cast<long long>(k + p + 1);
-                    sw[j] = p;
-                    if (p != j) {
-                        const int t = perm[k + j];
-                        perm[k + j] = perm[k + p];
-                        perm[k + p] = t;
-                    }
-                }
-                __syncthreads();
-                const T d = P[j][j];
-                if (d == T(0)) {
-                    if (tid == 0 && s_info == 0) s_info = k + j + 1;
-                } else if (tid > j && tid < m) {
-                    const bool use_recip = fabs(static_cast<double>(d)) >= static_cast<double>(TypeMin<T>::v);
-                    const T rcp = use_recip ? T(1) / d : T(1);
-                    const T l = use_recip ? P[tid][j] * rcp : P[tid][j] / d;
-                    P[tid][j] = l;
-                    for (int c = j + 1; c < w; ++c) P[tid][c] = P[tid][c] - l * P[j][c];
-                }
-                __syncthreads();
-            }
-            for (int idx = tid; idx < m * w; idx += kBatThreads) {
-                const int r = idx / w, c = idx - (idx / w) * w;
-                A[static_cast<long long>(k + r) * n + k + c] = P[r][c];
-            }
-            // the panel's interchanges on every other column (one column per thread)
-            for (int c = tid; c < n; c += kBatThreads) {
-                if (c >= k && c < k + w) continue;
-                for (int j = 0; j < w; ++j) {
-                    const int p = sw[j];
-                    if (p != j) {
-                        T* x = A + static_cast<long long>(k + j) * n + c;
-                        T* y = A + static_cast<long long>(k + p) * n + c;
-                        const T t = *x;
-                        *x = *y;
-                        *y = t;
-                    }
+                    for (int q2 = 0; q2 < 2 * NB; ++q2)
+                        v[q2] = (q2 < cnt) ? A[static_cast<long long>(src[q2]) * n + c] : T(0);
+#pragma unroll
+                    for (int q2 = 0; q2 < 2 * NB; ++q2)
+                        if (q2 < cnt) A[static_cast<long long>(dst[q2]) * n + c] = v[q2];
                 }
             }
             __syncthreads();
             if (k + w >= n) break;
-            // U12 = L11^-1 A12, one column per thread, rows in the reference's order
-            for (int c = k + w + tid; c < n; c += kBatThreads) {
+            // U12 = L11^-1 A12: one column per thread (n <= 256 = threads), the column in
+            // registers, rows in the reference's TRSM order
+            const int mr = m - w, nc = n - k - w;
+            if (tid < nc) {
+                T* const col = A + static_cast<long long>(k) * n + k + w + tid;
                 T u[NB];
 #pragma unroll
-                for (int i = 0; i < NB; ++i) u[i] = (i < w) ? A[static_cast<long long>(k + i) * n + c] : T(0);
+                for (int i = 0; i < NB; ++i) u[i] = (i < w) ? col[static_cast<long long>(i) * n] : T(0);
 #pragma unroll
-                for (int i = 1; i < NB; ++i)
+                for (int i = 1; i < NB; ++i) {
+                    if (i < w) {
 #pragma unroll
-                    for (int p = 0; p < i; ++p) u[i] = u[i] - P[i][p] * u[p];
+                        for (int p = 0; p < i; ++p) u[i] = u[i] - P[i][p] * u[p];
+                    }
+                    // one row of L11 in flight at a time: the scheduler would otherwise issue
+                    // all NB^2/2 LDS reads up front and run out of registers
+                    asm volatile("" ::: "memory");
+                }
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
-                    if (i < w) A[static_cast<long long>(k + i) * n + c] = u[i];
+                    if (i < w) col[static_cast<long long>(i) * n] = u[i];
             }
             __syncthreads();
             // A22 -= L21 U12: 64 x 64 per pass, 4 x 4 per thread
-            const int mr = m - w, nc = n - k - w;
             const int ty = tid >> 4, tx = tid & 15;
             for (int cb = 0; cb < nc; cb += UC) {
                 for (int idx = tid; idx < NB * UC; idx += kBatThreads) {
@@ -206,13 +252,20 @@ __global__ __launch_bounds__(kBatThreads) void batched_gesv_kernel(T* __restrict
                 }
                 __syncthreads();
                 for (int rb = 0; rb < mr; rb += 64) {
-                    T acc[4][4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) acc[i][jj] = T(0);
+                    T acc[4][4], cv[4][4];
                     const int r0 = rb + ty * 4;
+                    // C first: its global latency overlaps the products below
 #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const T* crow = A + static_cast<long long>(k + w + r0 + i) * n + k + w + cb + tx * 4;
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            acc[i][jj] = T(0);
+                            cv[i][jj] = (r0 + i < mr && cb + tx * 4 + jj < nc) ? crow[jj] : T(0);
+                        }
+                    }
+                    // partial unroll: a full one hoists every p's LDS operands and spills
+#pragma unroll 2
                     for (int p = 0; p < NB; ++p) {
                         T av[4], bw4[4];
 #pragma unroll
@@ -227,10 +280,10 @@ __global__ __launch_bounds__(kBatThreads) void batched_gesv_kernel(T* __restrict
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         if (r0 + i >= mr) continue;
-                        T* crow = A + static_cast<long long>(k + w + r0 + i) * n + k + w + cb;
+                        T* crow = A + static_cast<long long>(k + w + r0 + i) * n + k + w + cb + tx * 4;
 #pragma unroll
                         for (int jj = 0; jj < 4; ++jj)
-                            if (cb + tx * 4 + jj < nc) crow[tx * 4 + jj] = crow[tx * 4 + jj] - acc[i][jj];
+                            if (cb + tx * 4 + jj < nc) crow[jj] = cv[i][jj] - acc[i][jj];
                     }
                 }
                 __syncthreads();
