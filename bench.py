@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--batch-count", type=int, default=10000)
     p.add_argument("--batch-replays", type=int, default=20)
+    p.add_argument("--batch-mixed-count", type=int, default=64,
+                   help="systems of the batched mixed-precision workload (0 = skip)")
     p.add_argument("--mixed-update", type=int, default=3,
                    help="trailing update of the mixed factorisation: 0/1/2 = 1/2/3 bf16 planes, 3 = FP32 MFMA")
     return p.parse_args()
@@ -220,6 +222,58 @@ def batched_bench(torch, count, replays, seed, dev):
             "flops": flops, **out}
 
 
+def batched_mixed_bench(torch, count, seed, dev):
+    """The batched mixed-precision workload: `count` FP64 systems, n log-uniform on
+    [256, 4096], each U diag(s) V^T with kappa log-uniform on [1e1, 1e8]; trailing update
+    with two bf16 planes where kappa <= 1e4, FP32 matrix instructions otherwise; FP64
+    refinement to backward error <= 1e-14. Inputs generated on the GPU (not timed)."""
+    import numpy as np
+    from linearalgebra_amd import mixed as lam
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = np.exp(rng.uniform(np.log(256), np.log(4096), size=count)).astype(np.int64)
+    kappa = np.exp(rng.uniform(np.log(1e1), np.log(1e8), size=count))
+    a_parts, b_parts = [], []
+    for i in range(count):
+        ai, bi = conditioned_gpu(torch, int(n[i]), float(kappa[i]), seed + 1000 + 3 * i, dev)
+        a_parts.append(ai.reshape(-1))
+        b_parts.append(bi)
+    a = torch.cat(a_parts)
+    b = torch.cat(b_parts)
+    del a_parts, b_parts
+    modes = np.where(kappa <= 1e4, 1, 3).astype(np.int32)
+    lam.gesv_mixed_batched(a, n[:1], b, modes=modes[:1])  # warm-up (kernel set-up)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, stats = lam.gesv_mixed_batched(a, n, b, modes=modes, tol=1e-14, max_iter=30, restart=30)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    berrs = []
+    off_a = off_b = 0
+    for i in range(count):
+        k = int(n[i])
+        ai = a[off_a: off_a + k * k].view(k, k)
+        berrs.append(backward_error_gpu(torch, ai, x[off_b: off_b + k], b[off_b: off_b + k]))
+        off_a += k * k
+        off_b += k
+    flops = float((2.0 / 3.0 * n.astype(np.float64) ** 3).sum())
+    paths = [s["path_name"] for s in stats]
+    log(f"batched mixed: {count} systems in {dt:.3f} s")
+    return {"workload": f"{count} FP64 systems, n log-uniform [256,4096], kappa log-uniform [1e1,1e8], "
+                        "U diag(s) V^T; two bf16 planes where kappa <= 1e4, FP32 MFMA otherwise; one call",
+            "systems": count, "seconds": round(dt, 4), "solves_per_s": round(count / dt, 3),
+            "fp64_equivalent_tflops": round(flops / dt / 1e12, 3),
+            "max_backward_error": max(berrs), "paths": {p: paths.count(p) for p in sorted(set(paths))},
+            "ir_iterations": {"min": min(s["ir_iterations"] for s in stats),
+                              "max": max(s["ir_iterations"] for s in stats)},
+            "gmres_iterations_max": max(s["gmres_iterations"] for s in stats),
+            "per_system": [{"n": int(n[i]), "kappa": float(f"{kappa[i]:.3e}"), "mode": int(modes[i]),
+                            "path": stats[i]["path_name"], "ir": stats[i]["ir_iterations"],
+                            "gmres": stats[i]["gmres_iterations"], "berr": float(f"{berrs[i]:.3e}")}
+                           for i in range(min(count, 16))],
+            "note": "sequence of mixed solves inside one C call, not a fused kernel; per_system lists the first 16"}
+
+
 # ----------------------------------------------------------------------------- CPU baseline
 
 def cpu_baseline(n_cpu, n_target, seed):
@@ -366,6 +420,10 @@ def main():
     if not args.no_batched:
         batched = batched_bench(torch, args.batch_count, args.batch_replays, args.seed, dev)
 
+    batched_mixed = None
+    if args.batch_mixed_count > 0:
+        batched_mixed = batched_mixed_bench(torch, args.batch_mixed_count, args.seed, dev)
+
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline at n={args.cpu_n}")
@@ -408,6 +466,7 @@ def main():
         "phases_mixed": phases_mixed,
         "phases_plain_fp64": phases_plain,
         "batched": batched,
+        "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,
     }
     if not (berr <= 1e-14):

@@ -154,6 +154,18 @@ LRN_API int lrn_batched_gesv_f32(float* a, const int64_t* a_offset, int64_t* piv
  * used by the autotuner. Process-wide. */
 LRN_API int lrn_set_batched_grid(int workgroups);
 
+/* Many mixed-precision solves in one call (the batched mixed-precision form): system i
+ * (order n[i], FP64 A_i at a + a_offset[i], b_i at b + b_offset[i], solution to
+ * x + x_offset[i]) is solved as lrn_gesv_mixed_f64 would, in order, on one stream, with
+ * its own trailing-update mode (update_mode[i] as lrn_set_mixed_update; NULL = the
+ * process-wide mode) and its own statistics in stats[i]. `a_offset`, `n`, `b_offset`,
+ * `x_offset`, `update_mode` and `stats` are HOST arrays; a, b, x device memory. The
+ * workspace is sized for the largest order (lrn_gesv_mixed_workspace_size_f64).
+ * SYNCHRONOUS. */
+LRN_API int lrn_gesv_mixed_batched_f64(const double* a, const int64_t* a_offset, const int64_t* n, const double* b,
+    const int64_t* b_offset, double* x, const int64_t* x_offset, int64_t count, const int32_t* update_mode, double tol,
+    int max_iter, int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream);
+
 /* Device tuning knobs (outer block, panel width, panel workgroups) for the autotuner. A
  * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);

@@ -39,6 +39,8 @@ SIGNATURES = {
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
     "lrn_set_mixed_update": [_int],
+    "lrn_gesv_mixed_batched_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, ctypes.c_double, _int, _int, _vp, _sz,
+                                   _vp, _vp],
     "lrn_batched_gesv_workspace_size": [_i64, ctypes.POINTER(_sz)],
     "lrn_batched_gesv_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp],
     "lrn_batched_gesv_f32": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp],

@@ -426,17 +426,17 @@ int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long lon
     return LRN_OK;
 }
 
-// The large trailing update: FP64 / FP32 matrix-core GEMM, or bf16 operands with FP32
-// accumulation for the low-precision factorisation of the mixed solver.
+// The large trailing update: FP64 / FP32 matrix-core GEMM, or, for the low-precision
+// factorisation of the mixed solver (upd = 0, 1, 2), 1-3 bf16 planes with FP32 accumulation.
+// upd < 0 or 3: the GEMM of the data type.
 template <typename T>
 int launch_trailing(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M, long long N,
-    long long K, bool lowp, hipStream_t s)
+    long long K, int upd, hipStream_t s)
 {
     if constexpr (std::is_same<T, float>::value) {
-        const int mode = g_mixed_update.load();
-        if (lowp && mode <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, mode + 1);
+        if (upd >= 0 && upd <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, upd + 1);
     }
-    (void)lowp;
+    (void)upd;
     return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s);
 }
 
@@ -561,7 +561,7 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
 // The panel and GEMM streams are restricted to disjoint sets of compute units so the panel's
 // cooperative workgroups always find room. Nothing waits on the host.
 template <typename T, int W>
-int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, bool lowp = false)
+int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd = -1)
 {
     const long long lda = n;
     const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
@@ -585,7 +585,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
             if (KB >= n) break;
             LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
-                n - KB, B, lowp, s));
+                n - KB, B, upd, s));
             LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s)));
             continue;
         }
@@ -597,7 +597,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         if (far0 < n) {
             LRN_TRY((u12_columns<T, W>(A, n, K, B, far0, n, side->gemm)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
-                n - far0, B, lowp, side->gemm));
+                n - far0, B, upd, side->gemm));
         }
         LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
         // next block column: caller stream
@@ -606,7 +606,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
             LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, s));
             LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, s)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
-                B, lowp, s));
+                B, upd, s));
             LRN_HIP(hipEventRecord(side->ready, s));
             LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
             LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel)));
@@ -1002,8 +1002,9 @@ bool mixed_verbose()
 }
 
 int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, double tol, int max_iter, int m,
-    void* ws, size_t ws_size, lrn_mixed_stats* stats, hipStream_t s)
+    void* ws, size_t ws_size, lrn_mixed_stats* stats, hipStream_t s, int upd = -1)
 {
+    if (upd < 0 || upd > 3) upd = g_mixed_update.load();
     const bool fallback = ws_size >= mixed_ws_bytes(n, m, true);
     if (ws_size < mixed_ws_bytes(n, m, false)) return -9;
     Arena ar(ws, ws_size);
@@ -1027,7 +1028,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     const long long nn = n * n;
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(nn)), dim3(256), 0, s, A, L.a32, nn);
     LRN_HIP(hipGetLastError());
-    LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, true)));
+    LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd)));
     LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
     LRN_HIP(hipEventRecord(e1, s));
     int info = 0, err = 0;
@@ -1330,6 +1331,33 @@ LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, doub
     if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
     return gesv_mixed_impl(a, n, b, x, tol, max_iter, gmres_restart, workspace, workspace_size, stats,
         static_cast<hipStream_t>(stream));
+}
+
+LRN_API int lrn_gesv_mixed_batched_f64(const double* a, const int64_t* a_offset, const int64_t* n, const double* b,
+    const int64_t* b_offset, double* x, const int64_t* x_offset, int64_t count, const int32_t* update_mode, double tol,
+    int max_iter, int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream)
+{
+    if (count < 0) return -8;
+    if (count == 0) return LRN_OK;
+    if (a == nullptr || a_offset == nullptr) return -1;
+    if (n == nullptr) return -3;
+    if (b == nullptr || b_offset == nullptr) return -4;
+    if (x == nullptr || x_offset == nullptr) return -6;
+    if (!(tol > 0.0)) return -10;
+    if (max_iter <= 0) return -11;
+    if (gmres_restart <= 0 || gmres_restart > 200) return -12;
+    if (workspace == nullptr) return -13;
+    if (stats == nullptr) return -15;
+    for (int64_t i = 0; i < count; ++i)
+        if (n[i] <= 0 || n[i] > kMaxOrder || (update_mode != nullptr && (update_mode[i] < 0 || update_mode[i] > 3)))
+            return -3;
+    for (int64_t i = 0; i < count; ++i) {
+        const int st = gesv_mixed_impl(a + a_offset[i], n[i], b + b_offset[i], x + x_offset[i], tol, max_iter,
+            gmres_restart, workspace, workspace_size, &stats[i], static_cast<hipStream_t>(stream),
+            update_mode != nullptr ? update_mode[i] : -1);
+        if (st != LRN_OK) return st;
+    }
+    return LRN_OK;
 }
 
 LRN_API int lrn_batched_gesv_workspace_size(int64_t count, size_t* bytes)

@@ -225,7 +225,9 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
                 }
             }
         }
-        wave_argmax(bv, bi);
+        // lanes hold consecutive row ranges, so the lowest lane with the maximum holds the
+        // lowest such row: DPP maximum + ballot instead of a shuffle butterfly
+        wave_argmax_ordered(bv, bi);
         if (lane == 0) {
             red_v[wid] = bv;
             red_i[wid] = bi;
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
                     gi = cand_idx[lane];
                     gv = pivot_key(cand[lane * W + j], gi == i);
                 }
-                wave_argmax(gv, gi);
+                wave_argmax_ordered(gv, gi);  // lane b holds workgroup b's row range
                 if (lane == 0) {
                     win_row = gi;
                     win_b = static_cast<int>((gi - k) / R);

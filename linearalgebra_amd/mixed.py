@@ -1,13 +1,15 @@
 """Mixed-precision FP64 solve (build-owned extension of linear_solver, no reference counterpart).
 
-A is rounded to FP32 and factored with FP32 panels and bf16 matrix-core trailing updates
-(FP32 accumulation); LU-IR, then GMRES-IR, then (optionally) a plain FP64 LU recover an
-FP64-level normwise backward error. All compute is in liblinalg_rocm_native.so.
+A is rounded to FP32 and factored with FP32 panels and matrix-core trailing updates (FP32
+instructions, or FP32 operands split into 1-3 bf16 planes; FP32 accumulation); LU-IR, then
+GMRES-IR, then (optionally) a plain FP64 LU recover an FP64-level normwise backward error.
+All compute is in liblinalg_rocm_native.so.
 """
 from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -70,4 +72,37 @@ def gesv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, 
     return x, st.as_dict()
 
 
-__all__ = ["gesv_mixed", "set_mixed_update", "workspace_size", "MixedStats", "LinalgError"]
+def gesv_mixed_batched(a: torch.Tensor, n, b: torch.Tensor, modes=None, tol: float = 1e-14, max_iter: int = 30,
+                       restart: int = 30, fallback: bool = True, stream=None):
+    """Mixed-precision solves of many systems packed back to back: ``a`` holds the row-major
+    A_i (n_i^2 values each), ``b`` the b_i (n_i values each), both FP64 device tensors.
+    ``modes`` (optional): per-system trailing-update mode (see set_mixed_update).
+    Returns (x packed like b, list of per-system stats dicts). Synchronous."""
+    n = np.asarray(n, dtype=np.int64)
+    count = int(n.size)
+    if a.dtype != torch.float64 or b.dtype != torch.float64 or not a.is_cuda or not b.is_cuda:
+        raise ValueError("gesv_mixed_batched: a and b must be float64 device tensors")
+    a_off = np.zeros(count, dtype=np.int64)
+    b_off = np.zeros(count, dtype=np.int64)
+    if count:
+        a_off[1:] = np.cumsum(n * n)[:-1]
+        b_off[1:] = np.cumsum(n)[:-1]
+        if a.numel() < int((n * n).sum()) or b.numel() < int(n.sum()):
+            raise ValueError("gesv_mixed_batched: a or b smaller than the packed systems")
+    x = torch.empty_like(b)
+    ws = torch.empty(workspace_size(int(n.max()) if count else 1, restart, fallback), dtype=torch.uint8,
+                     device=a.device)
+    st = (MixedStats * max(count, 1))()
+    md = None
+    if modes is not None:
+        md = np.ascontiguousarray(modes, dtype=np.int32)
+    code = _lib.load().lrn_gesv_mixed_batched_f64(
+        _ptr(a), a_off.ctypes.data_as(ctypes.c_void_p), n.ctypes.data_as(ctypes.c_void_p), _ptr(b),
+        b_off.ctypes.data_as(ctypes.c_void_p), _ptr(x), b_off.ctypes.data_as(ctypes.c_void_p), count,
+        md.ctypes.data_as(ctypes.c_void_p) if md is not None else None, float(tol), int(max_iter), int(restart),
+        _ptr(ws), ws.numel(), ctypes.cast(st, ctypes.c_void_p), _stream_ptr(stream))
+    _check(code, "gesv_mixed_batched")
+    return x, [st[i].as_dict() for i in range(count)]
+
+
+__all__ = ["gesv_mixed", "gesv_mixed_batched", "set_mixed_update", "workspace_size", "MixedStats", "LinalgError"]

@@ -114,3 +114,38 @@ def test_gesv_mixed_diag_dominant_lu_ir(dev):
     x, st = la.gesv_mixed(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev))
     assert st["path"] == 0
     assert O.backward_error(a, x.cpu().numpy(), b) <= 2e-14
+
+
+def test_gesv_mixed_batched_uneven(dev):
+    """Many systems of uneven order and condition in one call, each with its own update
+    mode (two bf16 planes where kappa <= 1e4, FP32 beyond): every backward error reaches
+    the FP64 level and every forward error stays within kappa * n * eps of the oracle."""
+    import torch
+    import linearalgebra_amd as la
+    from linearalgebra_amd import mixed as lam
+    rng = np.random.Generator(np.random.PCG64(44))
+    count = 10
+    n = np.exp(rng.uniform(np.log(64), np.log(1024), size=count)).astype(np.int64)
+    kappa = np.exp(rng.uniform(np.log(1e1), np.log(1e8), size=count))
+    a = [O.conditioned(int(k), float(c), seed=700 + i) for i, (k, c) in enumerate(zip(n, kappa))]
+    b = [O.gaussian(int(k), 1, seed=800 + i).ravel() for i, k in enumerate(n)]
+    modes = np.where(kappa <= 1e4, 1, 3).astype(np.int32)
+    ta = torch.from_numpy(np.concatenate([x.ravel() for x in a])).to(dev)
+    tb = torch.from_numpy(np.concatenate(b)).to(dev)
+    x, stats = lam.gesv_mixed_batched(ta, n, tb, modes=modes, tol=1e-14, max_iter=30, restart=40)
+    xs = x.cpu().numpy()
+    off = 0
+    for i in range(count):
+        k = int(n[i])
+        xi = xs[off: off + k]
+        off += k
+        berr = O.backward_error(a[i], xi, b[i])
+        _, _, x_ref, _ = O.gesv(a[i], b[i])
+        fe = O.forward_error(xi, x_ref)
+        st = stats[i]
+        print(f"n={k} kappa={kappa[i]:.1e} mode={modes[i]} path={st['path_name']} ir={st['ir_iterations']} "
+              f"gmres={st['gmres_iterations']} berr={berr:.2e} fwd={fe:.2e}", flush=True)
+        assert berr <= 2e-14
+        assert st["path"] in (0, 1, 2)
+        assert fe <= max(1e-12, kappa[i] * k * np.finfo(np.float64).eps)
+    assert la is not None
