@@ -100,6 +100,16 @@ LRN_API int lrn_getrf_async_f64(double* m, int64_t lda, int64_t* pivot, int* inf
 LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
     size_t workspace_size, void* stream);
 
+/* ---- FP32 LU with tournament pivoting (no reference counterpart) -----------------------
+ * The factorisation used inside the mixed solver: same layout, pivot convention (1-based
+ * sequential interchanges) and info as lu_decomposition, but the pivot rows of each
+ * 32-column panel are chosen by a reduction tree of partial-pivoting eliminations over
+ * blocks of rows (communication-avoiding LU), so they need not equal partial pivoting's.
+ * info is written to DEVICE memory. ASYNCHRONOUS. */
+LRN_API int lrn_getrf_tournament_workspace_size_f32(int64_t lda, size_t* bytes);
+LRN_API int lrn_getrf_tournament_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream);
+
 /* ---- mixed-precision solve with iterative refinement (no reference counterpart) --------
  * Solves A x = b (FP64 A and b, one right-hand side) to FP64-level normwise backward error
  * with a low-precision factorisation: A is rounded to FP32, factored with FP32 panels and
@@ -127,6 +137,23 @@ typedef struct lrn_mixed_stats {
  * 3 = FP32 matrix instructions (the default). Modes 2 and 3 give FP32-level factors, which
  * LU-IR needs at kappa ~ 1e6; modes 0 and 1 rely on GMRES-IR there. */
 LRN_API int lrn_set_mixed_update(int mode);
+/* Panel of the FP32 factorisation inside the mixed solver (process-wide, no reference
+ * counterpart): 1 = tournament pivoting (the pivot rows of each 32-column panel are chosen
+ * by a reduction tree of partial-pivoting eliminations over row blocks, so the panel needs
+ * no per-column exchange between workgroups), 0 = the partial-pivoting panel of the plain
+ * path (the default). Either way the factors only precondition the FP64 refinement. */
+LRN_API int lrn_set_mixed_panel(int tournament);
+/* Triangular solves of the mixed solver's refinement (FP32 factors applied in FP64,
+ * process-wide, no reference counterpart): 1 = 256-row workgroups chained through tagged
+ * hand-offs, 0 = the 64-row kernel of the plain single-RHS solve (the default). */
+LRN_API int lrn_set_refine_solver(int chain);
+/* The refinement's preconditioner on its own (no reference counterpart): b <- U^-1 L^-1 P b
+ * with FP32 factors and pivots (as from lrn_getrf_tournament_f32 or lrn_getrf_async_f32)
+ * applied in FP64 arithmetic to one FP64 right-hand side; the kernel is the one selected by
+ * lrn_set_refine_solver. Device pointers. ASYNCHRONOUS. */
+LRN_API int lrn_getrs_f32f64_workspace_size(int64_t n, size_t* bytes);
+LRN_API int lrn_getrs_f32f64(const float* lu, int64_t n, const int64_t* pivot, double* b, void* workspace,
+    size_t workspace_size, void* stream);
 LRN_API int lrn_gesv_mixed_workspace_size_f64(int64_t n, int gmres_restart, int fallback, size_t* bytes);
 LRN_API int lrn_gesv_mixed_f64(const double* a, int64_t n, const double* b, double* x, double tol, int max_iter,
     int gmres_restart, void* workspace, size_t workspace_size, lrn_mixed_stats* stats, void* stream);

@@ -30,10 +30,12 @@
 #include "lrn_getrs_kernels.hpp"
 #include "lrn_panel.hpp"
 #include "lrn_gemm_f64.hpp"
+#include "lrn_gemm_f32.hpp"
 #include "lrn_gemm_bf16.hpp"
 #include "lrn_mixed.hpp"
 #include "lrn_trace.hpp"
 #include "lrn_batched.hpp"
+#include "lrn_calu.hpp"
 
 namespace lrn {
 namespace {
@@ -115,6 +117,7 @@ template <typename T> struct GetrsLayout {
     int* err;
     T* y;
     Chunk* xch;  // single right-hand side: tagged exchange of solved values, one per row
+    int* maps;   // net row moves of every 64-pivot block (permutation from the pivots)
 };
 
 long long solve_blocks(long long n) { return (n + kSolveTB - 1) / kSolveTB; }
@@ -129,6 +132,7 @@ template <typename T> GetrsLayout<T> carve_getrs(Arena& ar, long long n, long lo
     L.err = ar.next<int>(64);
     L.y = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(std::min<long long>(nrhs, kSolveMaxCols)));
     L.xch = ar.next<Chunk>(static_cast<size_t>(n));
+    L.maps = ar.next<int>(static_cast<size_t>((n + 63) / 64) * kPermMapInts);
     return L;
 }
 
@@ -299,6 +303,36 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     return LRN_OK;
 }
 
+// Tournament-pivoted panel of the FP32 factorisation (lrn_calu.hpp): selection (one launch,
+// the last workgroup to arrive picks the pivot rows) and application (one launch).
+template <int W>
+int launch_calu(float* A, long long lda, long long k, long long m, int w, long long* piv, CaluWs<float> cw,
+    int* rowmap, unsigned& cbase, hipStream_t s)
+{
+    const long long P = (m + kCaluRowsPerWG - 1) / kCaluRowsPerWG;
+    if (P > kCaluMaxP || w > W || w > m) return LRN_E_UNSUPPORTED;
+    cw.rowmap = rowmap;
+    TraceScope ts(kTrPanel, panel_flops(m, w), s);
+    const bool vec4 = w == W && lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    const dim3 sg(static_cast<unsigned>(P)), ag(static_cast<unsigned>((m + 255) / 256));
+    if (vec4)
+        hipLaunchKernelGGL((calu_select_kernel<W, true>), sg, dim3(kCaluThreads), 0, s, A, lda, k, m, w,
+            static_cast<int>(P), cbase, piv, cw);
+    else
+        hipLaunchKernelGGL((calu_select_kernel<W, false>), sg, dim3(kCaluThreads), 0, s, A, lda, k, m, w,
+            static_cast<int>(P), cbase, piv, cw);
+    LRN_HIP(hipGetLastError());
+    if (P > 1) cbase += static_cast<unsigned>(P);
+    if (vec4)
+        hipLaunchKernelGGL((calu_apply_kernel<W, true>), ag, dim3(256), 0, s, A, lda, k, m, w, cw.lu11, cw.top, cw.uinv,
+            rowmap);
+    else
+        hipLaunchKernelGGL((calu_apply_kernel<W, false>), ag, dim3(256), 0, s, A, lda, k, m, w, cw.lu11, cw.top,
+            cw.uinv, rowmap);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
 template <typename T>
 int launch_laswp(T* A, long long lda, const int* rowmap, int W2, long long c0a, long long c1a, long long c0b,
     long long c1b, hipStream_t s)
@@ -383,13 +417,35 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
         LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni));          // bottom strip
         return LRN_OK;
     } else {
-        constexpr int BM = 128, BN = 128, BK = 16;
-        const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-        if (tm * tn > 0x7fffffffll) return LRN_E_UNSUPPORTED;
-        TraceScope ts(kTrGemmF32, 2.0 * M * N * K, s);
-        hipLaunchKernelGGL((gemm_sub_kernel<T, BM, BN, BK>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s, A,
-            lda, B, ldb, C, ldc, static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), static_cast<int>(tn));
-        LRN_HIP(hipGetLastError());
+        // FP32: the same interior / ragged-strip split on the double-buffered kernel
+        const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0) &&
+                             (lda % 4 == 0) && (ldb % 4 == 0) && (K % kG32BK == 0);
+        const long long Mi = aligned ? M / kG32BM * kG32BM : 0;
+        const long long Ni = aligned ? N / kG32BN * kG32BN : 0;
+        auto launch = [&](bool check, const float* a, const float* b, float* c, long long m, long long nn) -> int {
+            if (m <= 0 || nn <= 0) return LRN_OK;
+            const long long tm = (m + kG32BM - 1) / kG32BM, tn = (nn + kG32BN - 1) / kG32BN;
+            if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
+            const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
+            int dev = 0;
+            LRN_HIP(hipGetDevice(&dev));
+            const long long cap = 2ll * device_props(dev).cus;
+            const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
+            TraceScope ts(kTrGemmF32, 2.0 * m * nn * K, s);
+            if (check)
+                hipLaunchKernelGGL(gemm_sub_f32_kernel<true>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
+                    static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
+                    static_cast<int>(tn), static_cast<int>(per_xcd));
+            else
+                hipLaunchKernelGGL(gemm_sub_f32_kernel<false>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
+                    static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
+                    static_cast<int>(tn), static_cast<int>(per_xcd));
+            LRN_HIP(hipGetLastError());
+            return LRN_OK;
+        };
+        LRN_TRY(launch(false, A, B, C, Mi, Ni));
+        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni));
+        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni));
         return LRN_OK;
     }
 }
@@ -398,6 +454,11 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
 // (three products), 2 = three bf16 planes (six products, FP32-level), 3 = FP32 MFMA (the
 // default: FP32-level factors, and on gfx950 faster than the six-product form of this build).
 std::atomic<int> g_mixed_update{3};
+
+// Panel of the low-precision factorisation: 1 = tournament pivoting, 0 = partial pivoting
+// with the cooperative panel of the plain path (the default: on gfx950 the tournament's
+// selection kernel is not yet faster than the cooperative panel).
+std::atomic<int> g_mixed_panel{0};
 
 // C[M x N] -= A * B, FP32 data, with `planes` bf16 planes per operand (1..3)
 int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long long ldb, float* C, long long ldc,
@@ -513,7 +574,7 @@ std::atomic<int> g_lookahead{0};
 // the block column (TRSM + GEMM restricted to columns [K, K+B)).
 template <typename T, int W>
 int factor_block_column(T* A, long long n, long long K, long long B, long long* piv, const PanelWs<T>& ws,
-    int* rowmaps, unsigned& base, hipStream_t s)
+    int* rowmaps, unsigned& base, hipStream_t s, const CaluWs<float>* calu, unsigned& cbase)
 {
     const long long lda = n;
     const long long KB = K + B;
@@ -523,7 +584,16 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
         const int w = static_cast<int>(std::min<long long>(W, KB - k));
         PanelWs<T> pw = ws;
         pw.rowmap = rowmaps + slot * rm_stride;
-        LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
+        if constexpr (std::is_same<T, float>::value && W == 32) {
+            if (calu != nullptr)
+                LRN_TRY(launch_calu<W>(A, lda, k, n - k, w, piv, *calu, pw.rowmap, cbase, s));
+            else
+                LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
+        } else {
+            (void)calu;
+            (void)cbase;
+            LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
+        }
         LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
         if (k + w < KB) {
             LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
@@ -561,19 +631,21 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
 // The panel and GEMM streams are restricted to disjoint sets of compute units so the panel's
 // cooperative workgroups always find room. Nothing waits on the host.
 template <typename T, int W>
-int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd = -1)
+int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd = -1,
+    const CaluWs<float>* calu = nullptr)
 {
     const long long lda = n;
     const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
     const int rm_stride = 1 + 4 * W;
     unsigned base = 0;
+    unsigned cbase = 0;  // arrival counter of the tournament panels (cleared with the panel state)
     AsideStream* side = nullptr;
     const bool lookahead = g_lookahead.load() != 0 && n > NB;
     if (lookahead) LRN_TRY(aside_stream(n > 16384 ? 64 : 32, &side));
     const int set_stride = kMaxInner * (1 + 4 * kMaxW);
     auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
     LRN_TRY(reset_panel_state(ws, s));
-    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s)));
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s, calu, cbase)));
     if (lookahead) LRN_HIP(hipEventRecord(side->done, s));
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
@@ -586,7 +658,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
             LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
                 n - KB, B, upd, s));
-            LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s)));
+            LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s, calu, cbase)));
             continue;
         }
         const long long B2 = std::min(NB, n - KB);  // next block column (0 at the end)
@@ -609,7 +681,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
                 B, upd, s));
             LRN_HIP(hipEventRecord(side->ready, s));
             LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
-            LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel)));
+            LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel, calu, cbase)));
             LRN_HIP(hipEventRecord(side->done, side->panel));
         }
         // the next step's caller-stream work needs this step's far update (its columns)
@@ -648,6 +720,11 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
     return ar.off;
 }
 
+// Single-RHS substitution with FP32 factors in FP64 arithmetic (the refinement's
+// preconditioner): 1 = 256-row chained kernel, 0 = the 64-row kernel (the default: the
+// 256-row kernel streams its factor rows through too few compute units to keep up).
+std::atomic<int> g_chain_solve{0};
+
 // Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
 // without touching B (LU mode after a singular factorisation).
 // Row permutation of the factorisation (from the pivots), into L.perm.
@@ -656,9 +733,12 @@ template <typename T> int getrs_perm(long long n, const long long* piv, const Ge
     const size_t perm_lds = (n <= kPermLdsMax) ? static_cast<size_t>(n) * sizeof(int) : 0;
     {
         int per_cu = 0;
-        LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(perm_from_pivots_kernel), perm_lds, 64, &per_cu));
+        LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(perm_compose_kernel), perm_lds, 256, &per_cu));
     }
-    hipLaunchKernelGGL(perm_from_pivots_kernel, dim3(1), dim3(64), perm_lds, s, piv, n, L.perm, L.perm_tmp);
+    const long long nblocks = (n + 63) / 64;
+    hipLaunchKernelGGL(pivot_block_maps_kernel, dim3(static_cast<unsigned>(nblocks)), dim3(64), 0, s, piv, n, L.maps);
+    LRN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(perm_compose_kernel, dim3(1), dim3(256), perm_lds, s, L.maps, n, nblocks, L.perm, L.perm_tmp);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -679,6 +759,27 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
             LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
             const double tri_bytes = (0.5 * static_cast<double>(n) * n) * sizeof(TF) + 4.0 * n * sizeof(T);
             TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
+            if constexpr (std::is_same<TF, float>::value && std::is_same<T, double>::value) {
+                if (g_chain_solve.load() != 0 && ldb == 1) {
+                    const unsigned nb256 = static_cast<unsigned>((n + kChainTB - 1) / kChainTB);
+                    const bool vec4 = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(LU) % 16 == 0);
+                    if (vec4) {
+                        hipLaunchKernelGGL((trsv_chain_kernel<T, true, true>), dim3(nb256), dim3(256), 0, s, LU, n, n,
+                            L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                        LRN_HIP(hipGetLastError());
+                        hipLaunchKernelGGL((trsv_chain_kernel<T, false, true>), dim3(nb256), dim3(256), 0, s, LU, n, n,
+                            L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                    } else {
+                        hipLaunchKernelGGL((trsv_chain_kernel<T, true, false>), dim3(nb256), dim3(256), 0, s, LU, n, n,
+                            L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                        LRN_HIP(hipGetLastError());
+                        hipLaunchKernelGGL((trsv_chain_kernel<T, false, false>), dim3(nb256), dim3(256), 0, s, LU, n, n,
+                            L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                    }
+                    LRN_HIP(hipGetLastError());
+                    continue;
+                }
+            }
             hipLaunchKernelGGL((trsv1_kernel<TF, T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n,
                 n, L.perm, B + c0, ldb, L.y, 1ll, L.xch, 1u, kSpinLimit, L.err, skip_dev);
             LRN_HIP(hipGetLastError());
@@ -750,6 +851,48 @@ int getrf_async_impl(T* m, int64_t lda, int64_t* pivot, int* info_dev, void* wor
     return LRN_OK;
 }
 
+struct TournamentLayout {
+    GetrfLayout<float> f;
+    CaluWs<float> calu;
+};
+
+TournamentLayout carve_tournament(Arena& ar)
+{
+    TournamentLayout L;
+    L.f = carve_getrf<float>(ar);
+    L.calu.cand = ar.next<int>(static_cast<size_t>(kCaluMaxP) * kMaxW);
+    L.calu.lu11 = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.top = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.uinv = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.ctr = L.f.ws.ctr == nullptr ? nullptr : L.f.ws.ctr + 32;
+    L.calu.info = L.f.ws.info;
+    L.calu.rowmap = nullptr;
+    return L;
+}
+
+size_t tournament_ws_bytes()
+{
+    Arena ar;
+    carve_tournament(ar);
+    return ar.off;
+}
+
+int getrf_tournament_impl(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace, size_t ws_size,
+    void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -2;
+    if (pivot == nullptr) return -3;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < tournament_ws_bytes()) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const TournamentLayout L = carve_tournament(ar);
+    LRN_TRY((getrf_blocked<float, 32>(m, lda, reinterpret_cast<long long*>(pivot), L.f.ws, s, -1, &L.calu)));
+    if (info_dev != nullptr) LRN_HIP(hipMemcpyAsync(info_dev, L.f.ws.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LRN_OK;
+}
+
 template <typename T>
 int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t ldb, int type, void* workspace,
     size_t ws_size, void* stream)
@@ -797,6 +940,7 @@ struct MixedLayout {
     float* a32;
     long long* piv;
     GetrfLayout<float> f;
+    CaluWs<float> calu;     // tournament-panel buffers of the FP32 factorisation
     GetrsLayout<double> g;  // FP64 substitution buffers (the FP32 factors are applied in FP64)
     double* r;
     double* w;
@@ -819,6 +963,13 @@ MixedLayout carve_mixed(Arena& ar, long long n, int m, bool fallback)
     L.a32 = ar.next<float>(static_cast<size_t>(n) * static_cast<size_t>(n));
     L.piv = ar.next<long long>(static_cast<size_t>(n));
     L.f = carve_getrf<float>(ar);
+    L.calu.cand = ar.next<int>(static_cast<size_t>(kCaluMaxP) * kMaxW);
+    L.calu.lu11 = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.top = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.uinv = ar.next<float>(static_cast<size_t>(kMaxW) * kMaxW);
+    L.calu.ctr = L.f.ws.ctr == nullptr ? nullptr : L.f.ws.ctr + 32;
+    L.calu.info = L.f.ws.info;
+    L.calu.rowmap = nullptr;
     L.g = carve_getrs<double>(ar, n, 1);
     L.r = ar.next<double>(static_cast<size_t>(n));
     L.w = ar.next<double>(static_cast<size_t>(n));
@@ -1028,7 +1179,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     const long long nn = n * n;
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(nn)), dim3(256), 0, s, A, L.a32, nn);
     LRN_HIP(hipGetLastError());
-    LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd)));
+    LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd, g_mixed_panel.load() ? &L.calu : nullptr)));
     LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
     LRN_HIP(hipEventRecord(e1, s));
     int info = 0, err = 0;
@@ -1219,6 +1370,20 @@ LRN_API int lrn_getrf_async_f32(float* m, int64_t lda, int64_t* pivot, int* info
     return getrf_async_impl<float>(m, lda, pivot, info_dev, workspace, workspace_size, stream);
 }
 
+LRN_API int lrn_getrf_tournament_workspace_size_f32(int64_t lda, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = tournament_ws_bytes();
+    return LRN_OK;
+}
+
+LRN_API int lrn_getrf_tournament_f32(float* m, int64_t lda, int64_t* pivot, int* info_dev, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return getrf_tournament_impl(m, lda, pivot, info_dev, workspace, workspace_size, stream);
+}
+
 LRN_API int lrn_linear_solver_workspace_size_f64(int64_t lda, int type, size_t* bytes)
 {
     if (lda <= 0) return -1;
@@ -1269,6 +1434,30 @@ LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, f
     int type, void* workspace, size_t workspace_size, void* stream)
 {
     return solver_impl<float>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_getrs_f32f64_workspace_size(int64_t n, size_t* bytes)
+{
+    if (n <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = getrs_ws_bytes<double>(n, 1);
+    return LRN_OK;
+}
+
+LRN_API int lrn_getrs_f32f64(const float* lu, int64_t n, const int64_t* pivot, double* b, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    if (lu == nullptr) return -1;
+    if (n <= 0) return -2;
+    if (pivot == nullptr) return -3;
+    if (b == nullptr) return -4;
+    if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || workspace_size < getrs_ws_bytes<double>(n, 1)) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, workspace_size);
+    const auto L = carve_getrs<double>(ar, n, 1);
+    LRN_TRY(getrs_perm<double>(n, reinterpret_cast<const long long*>(pivot), L, s));
+    return getrs_subst<double, float>(lu, n, b, 1, 1, L, nullptr, s);
 }
 
 LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void* b, int64_t ldb, void* c,
@@ -1432,6 +1621,18 @@ LRN_API int lrn_set_mixed_update(int mode)
 {
     if (mode < 0 || mode > 3) return -1;
     g_mixed_update.store(mode);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_mixed_panel(int tournament)
+{
+    g_mixed_panel.store(tournament ? 1 : 0);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_refine_solver(int chain)
+{
+    g_chain_solve.store(chain ? 1 : 0);
     return LRN_OK;
 }
 

@@ -26,6 +26,7 @@
 namespace lrn {
 
 constexpr int kG64BM = 128, kG64BN = 128, kG64BK = 16;
+typedef double g64v2 __attribute__((ext_vector_type(2)));  // native vector: stays in registers
 constexpr int kXcds = 8;
 constexpr int kGroupRows = 8;
 
@@ -52,7 +53,7 @@ __device__ __forceinline__ bool xcd_tile(int wg, int tm, int tn, int per_xcd, in
 // Global -> register staging of one K tile (BK = 16): 2 pairs of A and 2 pairs of B per thread.
 // kCheck selects the bounds-checked form for edge tiles; interior tiles use plain 16-byte loads.
 template <bool kCheck, bool kVec>
-__device__ __forceinline__ void g64_load(double2 (&ra)[2], double2 (&rb)[2], const double* __restrict__ A, long long lda,
+__device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const double* __restrict__ A, long long lda,
     const double* __restrict__ B, long long ldb, long long row0, long long col0, int k0, int M, int N, int K, int tid)
 {
 #pragma unroll
@@ -62,13 +63,13 @@ __device__ __forceinline__ void g64_load(double2 (&ra)[2], double2 (&rb)[2], con
         const long long gr = row0 + r;
         const int gk = k0 + c;
         if (!kCheck) {
-            ra[q] = *reinterpret_cast<const double2*>(A + gr * lda + gk);
+            ra[q] = *reinterpret_cast<const g64v2*>(A + gr * lda + gk);
         } else {
-            double2 v = {0.0, 0.0};
+            g64v2 v = {0.0, 0.0};
             if (gr < M) {
                 const double* p = A + gr * lda + gk;
                 if (kVec && gk + 1 < K) {
-                    v = *reinterpret_cast<const double2*>(p);
+                    v = *reinterpret_cast<const g64v2*>(p);
                 } else {
                     if (gk < K) v.x = p[0];
                     if (gk + 1 < K) v.y = p[1];
@@ -84,13 +85,13 @@ __device__ __forceinline__ void g64_load(double2 (&ra)[2], double2 (&rb)[2], con
         const int gk = k0 + r;
         const long long gc = col0 + c;
         if (!kCheck) {
-            rb[q] = *reinterpret_cast<const double2*>(B + static_cast<long long>(gk) * ldb + gc);
+            rb[q] = *reinterpret_cast<const g64v2*>(B + static_cast<long long>(gk) * ldb + gc);
         } else {
-            double2 v = {0.0, 0.0};
+            g64v2 v = {0.0, 0.0};
             if (gk < K) {
                 const double* p = B + static_cast<long long>(gk) * ldb + gc;
                 if (kVec && gc + 1 < N) {
-                    v = *reinterpret_cast<const double2*>(p);
+                    v = *reinterpret_cast<const g64v2*>(p);
                 } else {
                     if (gc < N) v.x = p[0];
                     if (gc + 1 < N) v.y = p[1];
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
 #pragma unroll
         for (int b = 0; b < CG; ++b) acc[a][b] = 0.0;
 
-    double2 ra[2], rb[2];
+    g64v2 ra[2], rb[2];
     const int nk = (K + BK - 1) / BK;
     g64_load<kCheck, false>(ra, rb, A, lda, B, ldb, row0, col0, 0, M, N, K, tid);
 
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
         {                                                                               \
             const int idx = tid + q * 512;                                              \
             const int r = idx >> 6, c = (idx & 63) * 2;                                 \
-            *reinterpret_cast<double2*>(&Bs[buf][r][c]) = rb[q];                        \
+            *reinterpret_cast<g64v2*>(&Bs[buf][r][c]) = rb[q];                        \
         }                                                                               \
     } while (0)
 

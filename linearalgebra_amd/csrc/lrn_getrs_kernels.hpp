@@ -17,35 +17,62 @@ constexpr int kSolveMaxCols = 64;     // right-hand sides handled per launch
 constexpr int kPermLdsMax = 36864;    // largest n whose permutation is built in LDS
 
 // perm[i] = row of the input that lands in row i after the interchanges
-// (i, piv[i]-1), i = 0..n-1, applied in order. One wave walks the pivots in blocks of 64:
-// each block's net moves come from panel_rowmap and are applied to the label array.
-__global__ __launch_bounds__(64) void perm_from_pivots_kernel(const long long* __restrict__ piv, long long n,
+// (i, piv[i]-1), i = 0..n-1, applied in order, computed in two parallel steps. Step 1: the net
+// row moves of every block of 64 pivots (panel_rowmap), all blocks at once, one wave each;
+// maps[blk] holds {count, dst[128], src[128]}. Step 2: one workgroup composes the block maps
+// in order on a label array (LDS for n <= kPermLdsMax), prefetching the next block's moves.
+constexpr int kPermMapInts = 1 + 4 * 64;
+
+__global__ __launch_bounds__(64) void pivot_block_maps_kernel(const long long* __restrict__ piv, long long n,
+    int* __restrict__ maps)
+{
+    __shared__ long long pivs[64];
+    const int lane = threadIdx.x;
+    const long long k = static_cast<long long>(blockIdx.x) * 64;
+    const int w = static_cast<int>(min(64ll, n - k));
+    if (lane < w) pivs[lane] = piv[k + lane] - 1;
+    __syncthreads();
+    panel_rowmap(pivs, k, w, 128, maps + static_cast<size_t>(blockIdx.x) * kPermMapInts);
+}
+
+__global__ __launch_bounds__(256) void perm_compose_kernel(const int* __restrict__ maps, long long n, long long nblocks,
     int* __restrict__ perm_out, int* __restrict__ perm_tmp)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const bool in_lds = n <= kPermLdsMax;
     int* perm = in_lds ? reinterpret_cast<int*>(lds_raw) : perm_tmp;
-    __shared__ long long pivs[64];
-    __shared__ int rowmap[1 + 4 * 64];
-    const int lane = threadIdx.x;
-    for (long long i = lane; i < n; i += 64) perm[i] = static_cast<int>(i);
-    __syncthreads();
-    for (long long k = 0; k < n; k += 64) {
-        const int w = static_cast<int>(min(64ll, n - k));
-        if (lane < w) pivs[lane] = piv[k + lane] - 1;
-        __syncthreads();
-        panel_rowmap(pivs, k, w, 128, rowmap);
-        __syncthreads();
-        const int cnt = rowmap[0];
-        int v0 = 0, v1 = 0;
-        if (lane < cnt) v0 = perm[rowmap[1 + 128 + lane]];
-        if (lane + 64 < cnt) v1 = perm[rowmap[1 + 128 + lane + 64]];
-        __syncthreads();
-        if (lane < cnt) perm[rowmap[1 + lane]] = v0;
-        if (lane + 64 < cnt) perm[rowmap[1 + lane + 64]] = v1;
-        __syncthreads();
+    const int t = threadIdx.x;
+    for (long long i = t; i < n; i += 256) perm[i] = static_cast<int>(i);
+    // thread t < 128 handles move t of a block
+    int cnt = 0, dst = 0, src = 0;
+    if (nblocks > 0) {
+        cnt = maps[0];
+        if (t < 128) {
+            dst = maps[1 + t];
+            src = maps[1 + 128 + t];
+        }
     }
-    for (long long i = lane; i < n; i += 64) perm_out[i] = perm[i];
+    __syncthreads();
+    for (long long blk = 0; blk < nblocks; ++blk) {
+        int ncnt = 0, ndst = 0, nsrc = 0;
+        if (blk + 1 < nblocks) {
+            const int* rm = maps + static_cast<size_t>(blk + 1) * kPermMapInts;
+            ncnt = rm[0];
+            if (t < 128) {
+                ndst = rm[1 + t];
+                nsrc = rm[1 + 128 + t];
+            }
+        }
+        int v = 0;
+        if (t < cnt) v = perm[src];
+        __syncthreads();
+        if (t < cnt) perm[dst] = v;
+        __syncthreads();
+        cnt = ncnt;
+        dst = ndst;
+        src = nsrc;
+    }
+    for (long long i = t; i < n; i += 256) perm_out[i] = perm[i];
 }
 
 template <typename T> __device__ __forceinline__ T read_lane_t(T v, int lane);
@@ -314,6 +341,208 @@ __global__ __launch_bounds__(256) void trsv1_kernel(const TF* __restrict__ LU, l
         if (lane < rows) {
             Out[(r0 + lane) * ld_out] = x;
             st_chunk(xch + r0 + lane, static_cast<double>(x), static_cast<int>(r0 + lane), tag);
+        }
+    }
+}
+
+// Single right-hand side with FP32 factors applied in the working type T (the refinement of
+// the mixed solver): 256-row workgroups, so the chain of hand-offs between workgroups has
+// n/256 links instead of n/64.
+//  * off-diagonal part: the solved values arrive as tagged chunks in 64-row sub-blocks
+//    (wave 0 polls one chunk per lane). Wave w covers rows 64w..64w+63 of the block; lane l
+//    reads columns 4(l%16)..+3 of rows 4i + l/16 (i < 16), so every load instruction reads
+//    four full 256-byte row pieces (coalesced), one sub-block ahead of its use; each lane
+//    keeps 16 running partial sums, combined through LDS once all sub-blocks are in;
+//  * diagonal block: four 64-row sub-blocks solved in turn, each by one wave with its row of
+//    the diagonal sub-block in registers and lane reads for the solved values; the six
+//    off-diagonal 64 x 64 pieces inside the block are staged in LDS at the start, and after
+//    each sub-block the waves of the later sub-blocks take its contribution at once; every
+//    solved sub-block is published immediately, so the next workgroup starts on it while
+//    this one solves the rest.
+// Lower pass: rows in increasing order, unit diagonal. Upper pass: decreasing order,
+// q = x * (1/d) with one residual correction.
+constexpr int kChainTB = 256;
+constexpr int kChainSB = 64;
+
+__device__ __forceinline__ int chain_piece(int s, int sp, bool lower)
+{
+    // lower: (s, s') with s' < s; upper: (s, s') with s' > s, mirrored onto the same slots
+    const int a = lower ? s : 3 - s, c = lower ? sp : 3 - sp;
+    return a * (a - 1) / 2 + c;
+}
+
+template <typename T, bool kLower, bool kVec4>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const float* __restrict__ LU, long long lda, long long n,
+    const int* __restrict__ perm, const T* __restrict__ Bin, T* __restrict__ Out, Chunk* __restrict__ xch,
+    unsigned tag, unsigned spin_limit, int* __restrict__ err, const int* __restrict__ skip_if_nonzero)
+{
+    if (skip_if_nonzero != nullptr && *skip_if_nonzero != 0) return;
+    constexpr int TB = kChainTB, SB = kChainSB, NS = TB / SB;
+    __shared__ float Lp[6][SB][SB + 1];
+    __shared__ T Xs[2][SB];
+    __shared__ T Xd[NS][SB];
+    __shared__ T Pp[TB][17];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    const long long nblk = (n + TB - 1) / TB;
+    const long long b = kLower ? blockIdx.x : (nblk - 1 - blockIdx.x);
+    const long long r0 = b * TB;
+    const int rows = static_cast<int>(min(static_cast<long long>(TB), n - r0));
+    const int nsub = (rows + SB - 1) / SB;
+    const long long nsub_total = (n + SB - 1) / SB;
+    const bool row_ok = tid < rows;
+    const long long row = r0 + tid;
+    const float* lrow = LU + (row_ok ? row : r0) * lda;
+
+    for (int idx = tid; idx < 6 * SB * SB; idx += 256) {
+        const int piece = idx / (SB * SB);
+        const int rem = idx - piece * SB * SB;
+        const int rr = rem / SB, cc = rem - rr * SB;
+        // piece -> (a, c) with c < a in the mirrored numbering
+        const int a = piece < 1 ? 1 : (piece < 3 ? 2 : 3);
+        const int c = piece - a * (a - 1) / 2;
+        const int s = kLower ? a : 3 - a, sp = kLower ? c : 3 - c;
+        const long long gr = r0 + s * SB + rr, gc = r0 + sp * SB + cc;
+        Lp[piece][rr][cc] = (gr < n && gc < n) ? LU[gr * lda + gc] : 0.f;
+    }
+    float drow[SB];
+#pragma unroll
+    for (int p = 0; p < SB; ++p) {
+        const long long gc = r0 + wid * SB + p;
+        drow[p] = (row_ok && gc < n) ? lrow[gc] : 0.f;
+    }
+    T d_self = T(1), rcp_self = T(1);
+    if (!kLower && row_ok) {
+        d_self = static_cast<T>(lrow[row]);
+        rcp_self = T(1) / d_self;
+    }
+    T rhs = T(0);
+    if (row_ok) rhs = Bin[kLower ? static_cast<long long>(perm[row]) : row];
+
+    // off-diagonal sub-blocks, in the order they are solved; the row pieces alternate
+    // between two register buffers (one in use, the next one loading)
+    const long long jcount = kLower ? b * NS : (nsub_total - (b + 1) * NS);
+    const int rq = lane >> 4, cq = lane & 15;
+    T acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = T(0);
+    float la[16][4], lb[16][4];
+    auto sub_at = [&](long long q) { return kLower ? q : (nsub_total - 1 - q); };
+    auto load_piece = [&](float (&dst)[16][4], long long j) {
+        const long long c0 = j * SB + 4 * cq;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int lr = wid * SB + 4 * i + rq;
+            const bool ok = lr < rows;
+            const float* src = LU + (r0 + (ok ? lr : 0)) * lda + c0;
+            if (kVec4) {
+                // c0 + 3 < n whenever c0 < n: n is a multiple of 4 for this variant
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ok && c0 < n) v = *reinterpret_cast<const float4*>(src);
+                dst[i][0] = v.x;
+                dst[i][1] = v.y;
+                dst[i][2] = v.z;
+                dst[i][3] = v.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) dst[i][k] = (ok && c0 + k < n) ? src[k] : 0.f;
+            }
+        }
+    };
+    auto step = [&](long long q, const float (&cur)[16][4], float (&nxt)[16][4]) {
+        const long long j = sub_at(q);
+        const int buf = static_cast<int>(q & 1);
+        if (wid == 0) {
+            const long long g = j * SB + lane;
+            T v = T(0);
+            if (g < n) {
+                const Chunk* const pc[1] = {xch + g};
+                u32x4 c[1];
+                unsigned spins = 0;
+                for (;;) {
+                    ld_chunks<1>(c, pc);
+                    if (c[0].w == tag) break;
+                    ++spins;
+                    if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                v = static_cast<T>(chunk_value(c[0]));
+            }
+            Xs[buf][lane] = v;
+        }
+        __syncthreads();
+        if (q + 1 < jcount) load_piece(nxt, sub_at(q + 1));
+        T xv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) xv[k] = Xs[buf][4 * cq + k];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[i] = fma(static_cast<T>(cur[i][k]), xv[k], acc[i]);
+    };
+    if (jcount > 0) load_piece(la, sub_at(0));
+    for (long long q = 0; q < jcount; q += 2) {
+        step(q, la, lb);
+        if (q + 1 < jcount) step(q + 1, lb, la);
+    }
+    // combine the 16 partial sums of every row
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Pp[wid * SB + 4 * i + rq][cq] = acc[i];
+    __syncthreads();
+    T part = T(0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) part += Pp[tid][k];
+
+    // diagonal block, sub-block by sub-block
+    for (int si = 0; si < NS; ++si) {
+        const int s = kLower ? si : (NS - 1 - si);
+        if (s >= nsub) continue;
+        __syncthreads();
+        if (wid == s) {
+            T x = row_ok ? rhs - part : T(0);
+            const int live = rows - s * SB;  // rows of this sub-block
+            if (kLower) {
+#pragma unroll
+                for (int p = 0; p < SB; ++p) {
+                    if (p < live) {
+                        const T xp = read_lane_t<T>(x, p);
+                        if (lane > p) x = x - static_cast<T>(drow[p]) * xp;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int p = SB - 1; p >= 0; --p) {
+                    if (p < live) {
+                        if (lane == p) {
+                            const T q0 = x * rcp_self;
+                            x = fma(fma(-q0, d_self, x), rcp_self, q0);
+                        }
+                        const T xp = read_lane_t<T>(x, p);
+                        if (lane < p) x = x - static_cast<T>(drow[p]) * xp;
+                    }
+                }
+            }
+            Xd[s][lane] = row_ok ? x : T(0);
+            if (row_ok) {
+                Out[row] = x;
+                st_chunk(xch + row, static_cast<double>(x), static_cast<int>(row), tag);
+            }
+        }
+        __syncthreads();
+        const bool later = kLower ? (wid > s) : (wid < s);
+        if (later && wid < nsub) {
+            const int piece = chain_piece(wid, s, kLower);
+#pragma unroll
+            for (int e0 = 0; e0 < SB; e0 += 16) {
+#pragma unroll
+                for (int e = e0; e < e0 + 16; ++e) part = fma(static_cast<T>(Lp[piece][lane][e]), Xd[s][e], part);
+                asm volatile("" ::: "memory");
+            }
         }
     }
 }

@@ -50,6 +50,55 @@ def set_mixed_update(mode: int) -> None:
     _check(_lib.load().lrn_set_mixed_update(int(mode)), "set_mixed_update")
 
 
+def set_mixed_panel(tournament: bool) -> None:
+    """Panel of the FP32 factorisation inside the mixed solver (process-wide): True =
+    tournament pivoting, False = the partial-pivoting panel of the plain path (the default)."""
+    _check(_lib.load().lrn_set_mixed_panel(int(bool(tournament))), "set_mixed_panel")
+
+
+def getrf_tournament(m: torch.Tensor, pivot: torch.Tensor | None = None, stream=None):
+    """FP32 LU with tournament pivoting, in place (the factorisation of the mixed solver).
+    Returns (pivot, info) with 1-based sequential-interchange pivots; synchronises the stream
+    to read info."""
+    if m is None or m.dtype != torch.float32 or not m.is_cuda or m.dim() != 2 or m.shape[0] != m.shape[1]:
+        raise ValueError("getrf_tournament: m must be a square float32 device matrix")
+    if not m.is_contiguous():
+        raise ValueError("getrf_tournament: m must be contiguous (row-major, lda = n)")
+    n = m.shape[0]
+    if pivot is None:
+        pivot = torch.empty(n, dtype=torch.int64, device=m.device)
+    lib = _lib.load()
+    out = ctypes.c_size_t(0)
+    _check(lib.lrn_getrf_tournament_workspace_size_f32(int(n), ctypes.byref(out)), "getrf_tournament::workspace_size")
+    ws = torch.empty(int(out.value), dtype=torch.uint8, device=m.device)
+    info = torch.zeros(1, dtype=torch.int32, device=m.device)
+    _check(lib.lrn_getrf_tournament_f32(_ptr(m), int(n), _ptr(pivot), _ptr(info), _ptr(ws), ws.numel(),
+                                        _stream_ptr(stream)), "getrf_tournament")
+    return pivot, int(info.cpu().item())
+
+
+def set_refine_solver(chain: bool) -> None:
+    """Triangular solves of the refinement (process-wide): True = 256-row chained kernel,
+    False = the 64-row kernel of the plain single-RHS solve (the default)."""
+    _check(_lib.load().lrn_set_refine_solver(int(bool(chain))), "set_refine_solver")
+
+
+def getrs_f32f64(lu: torch.Tensor, pivot: torch.Tensor, b: torch.Tensor, stream=None) -> torch.Tensor:
+    """b <- U^-1 L^-1 P b in place: FP32 factors applied in FP64 to one FP64 right-hand side."""
+    if lu is None or lu.dtype != torch.float32 or not lu.is_cuda or lu.dim() != 2 or not lu.is_contiguous():
+        raise ValueError("getrs_f32f64: lu must be a contiguous square float32 device matrix")
+    n = lu.shape[0]
+    if b is None or b.dtype != torch.float64 or not b.is_cuda or not b.is_contiguous() or b.numel() != n:
+        raise ValueError("getrs_f32f64: b must be a contiguous float64 device vector of n")
+    lib = _lib.load()
+    out = ctypes.c_size_t(0)
+    _check(lib.lrn_getrs_f32f64_workspace_size(int(n), ctypes.byref(out)), "getrs_f32f64::workspace_size")
+    ws = torch.empty(int(out.value), dtype=torch.uint8, device=lu.device)
+    _check(lib.lrn_getrs_f32f64(_ptr(lu), int(n), _ptr(pivot), _ptr(b), _ptr(ws), ws.numel(), _stream_ptr(stream)),
+           "getrs_f32f64")
+    return b
+
+
 def gesv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, tol: float = 1e-14,
                max_iter: int = 20, restart: int = 30, workspace: torch.Tensor | None = None,
                fallback: bool = True, stream=None):
