@@ -147,6 +147,10 @@ LRN_API int lrn_set_mixed_panel(int tournament);
  * process-wide, no reference counterpart): 1 = 256-row workgroups chained through tagged
  * hand-offs, 0 = the 64-row kernel of the plain single-RHS solve (the default). */
 LRN_API int lrn_set_refine_solver(int chain);
+/* Kernel of the 32-column cooperative panel (process-wide, no reference counterpart; same
+ * pivots and factors either way): 0 = panel rows held in registers, two workgroup barriers
+ * per column (the default); 1 = panel rows in LDS with deferred updates. */
+LRN_API int lrn_set_panel_kernel(int kind);
 /* The refinement's preconditioner on its own (no reference counterpart): b <- U^-1 L^-1 P b
  * with FP32 factors and pivots (as from lrn_getrf_tournament_f32 or lrn_getrf_async_f32)
  * applied in FP64 arithmetic to one FP64 right-hand side; the kernel is the one selected by

@@ -37,6 +37,7 @@ SIGNATURES = {
     "lrn_getrf_tournament_f32": [_vp, _i64, _vp, _vp, _vp, _sz, _vp],
     "lrn_set_mixed_panel": [_int],
     "lrn_set_refine_solver": [_int],
+    "lrn_set_panel_kernel": [_int],
     "lrn_getrs_f32f64_workspace_size": [_i64, ctypes.POINTER(_sz)],
     "lrn_getrs_f32f64": [_vp, _i64, _vp, _vp, _vp, _sz, _vp],
     "lrn_debug_gemm_sub": [_int, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp],

@@ -212,6 +212,10 @@ int prepare_kernel(const void* f, size_t lds, int threads, int* per_cu)
 
 template <typename T> size_t panel_lds_bytes(int R, int W) { return static_cast<size_t>(R) * (W + 1) * sizeof(T); }
 
+// Panel kernel of the 32-column panels: 0 = register-resident rows (getf2_panel4_kernel, the
+// default), 1 = rows in LDS with deferred updates (getf2_panel3_kernel).
+std::atomic<int> g_panel_kind{0};
+
 int g_panel2_target_wg()
 {
     const int v = g_panel_max_wg.load();
@@ -234,7 +238,29 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     LRN_HIP(hipGetDevice(&dev));
     const DeviceProps& props = device_props(dev);
     if constexpr (W == 32) {
-        // one-round-trip cooperative panel (lrn_panel.hpp)
+        // register-resident cooperative panel (lrn_panel.hpp, getf2_panel4_kernel): one row per
+        // thread, so up to 64 x 256 = 16384 rows
+        const long long P4 = (m + kPanelThreads - 1) / kPanelThreads;
+        if (g_panel_kind.load() == 0 && P4 <= kPanel2MaxWG) {
+            const int P = static_cast<int>(P4);
+            Panel2Ws<T> pw;
+            pw.slots = static_cast<Chunk*>(ws.chunks);
+            pw.dslots = static_cast<Chunk*>(ws.dchunks);
+            pw.info = ws.info;
+            pw.err = ws.err;
+            pw.rowmap = ws.rowmap;
+            auto kern = getf2_panel4_kernel<T, W, 1>;
+            int per_cu = 0;
+            LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), 0, kPanelThreads, &per_cu));
+            if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
+            TraceScope ts(kTrPanel, panel_flops(m, w), s);
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(P)), dim3(kPanelThreads), 0, s, A, lda, k, m, w, P, base,
+                kSpinLimit, piv, pw);
+            LRN_HIP(hipGetLastError());
+            base += static_cast<unsigned>(w);
+            return LRN_OK;
+        }
+        // one-round-trip cooperative panel with the rows in LDS (getf2_panel3_kernel)
         int R, P;
         if (m <= 256) {
             R = std::max(next_pow2(m), 32);
@@ -355,6 +381,15 @@ int launch_laswp_multi(T* A, long long lda, const int* rowmaps, int nmaps, int m
 {
     const long long total = (c1a - c0a) + (c1b - c0b);
     if (total <= 0 || nmaps <= 0) return LRN_OK;
+    if (W2 <= 64 && nmaps <= 16 && nmaps * W2 <= kSwapBlkRows) {
+        // one gather and one scatter for all the maps
+        const long long blocks = (total + kSwapBlkCols - 1) / kSwapBlkCols;
+        TraceScope ts(kTrLaswp, 2.0 * W2 * nmaps * static_cast<double>(total) * sizeof(T), s);
+        hipLaunchKernelGGL(laswp_block_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, rowmaps,
+            nmaps, map_stride, W2, c0a, c1a, c0b, c1b);
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
     const long long blocks = (total + kSwapCols - 1) / kSwapCols;
     const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
     int per_cu = 0;
@@ -610,14 +645,23 @@ template <typename T, int W>
 int u12_columns(T* A, long long n, long long K, long long B, long long c0, long long c1, hipStream_t s)
 {
     const long long lda = n;
-    const long long KB = K + B;
     if (c1 <= c0) return LRN_OK;
-    for (long long k = K; k < KB; k += W) {
-        const int w = static_cast<int>(std::min<long long>(W, KB - k));
-        LRN_TRY((launch_trsm<T, W>(A, lda, k, w, c0, c1, s)));
-        LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda,
-            KB - (k + w), c1 - c0, w, s));
+    if (B > 256) {
+        const long long KB = K + B;
+        for (long long k = K; k < KB; k += W) {
+            const int w = static_cast<int>(std::min<long long>(W, KB - k));
+            LRN_TRY((launch_trsm<T, W>(A, lda, k, w, c0, c1, s)));
+            LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda,
+                KB - (k + w), c1 - c0, w, s));
+        }
+        return LRN_OK;
     }
+    // one launch for the whole block column
+    const long long blocks = (c1 - c0 + kU12TN - 1) / kU12TN;
+    TraceScope ts(kTrTrsm, (static_cast<double>(B) * B + 2.0 * B * static_cast<double>(c1 - c0)) * sizeof(T), s);
+    hipLaunchKernelGGL(u12_block_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, K,
+        static_cast<int>(B), c0, c1);
+    LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
 
@@ -1627,6 +1671,13 @@ LRN_API int lrn_set_mixed_update(int mode)
 LRN_API int lrn_set_mixed_panel(int tournament)
 {
     g_mixed_panel.store(tournament ? 1 : 0);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_panel_kernel(int kind)
+{
+    if (kind < 0 || kind > 1) return -1;
+    g_panel_kind.store(kind);
     return LRN_OK;
 }
 

@@ -395,6 +395,195 @@ __global__ __launch_bounds__(256) void trsm_llnu_kernel(T* __restrict__ A, long 
     }
 }
 
+// All row maps of a block column (nmaps maps, each the net moves of one inner panel, at most
+// W2 moves each, applied in order) on the columns [c0a, c1a) and [c0b, c1b) in one pass:
+// each workgroup takes a strip of kSwapBlkCols columns, gives every distinct row named by the
+// maps a slot (LDS hash table keyed by row), gathers those rows once, applies the maps one
+// after the other inside LDS, and scatters the rows back once: two memory round trips instead
+// of two per map. The result equals applying the maps in order (laswp_multi_kernel).
+constexpr int kSwapBlkCols = 16;
+constexpr int kSwapBlkRows = 512;  // distinct rows a block column's maps can name (2 * 256)
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_block_kernel(T* __restrict__ A, long long lda,
+    const int* __restrict__ rowmaps, int nmaps, int map_stride, int W2, long long c0a, long long c1a, long long c0b,
+    long long c1b)
+{
+    constexpr int SC = kSwapBlkCols, HS = 2048, MAXM = 16;
+    __shared__ T buf[kSwapBlkRows][SC + 1];
+    __shared__ int hkey[HS];
+    __shared__ int hslot[HS];
+    __shared__ int rowof[kSwapBlkRows];
+    __shared__ short sdst[MAXM][64];
+    __shared__ short ssrc[MAXM][64];
+    __shared__ int cnts[MAXM];
+    __shared__ int nslots;
+    const int tid = threadIdx.x;
+    const long long la = c1a - c0a;
+    const long long total = la + (c1b - c0b);
+    const long long s0 = static_cast<long long>(blockIdx.x) * SC;
+    for (int i = tid; i < HS; i += 256) hkey[i] = -1;
+    if (tid == 0) nslots = 0;
+    if (tid < nmaps) cnts[tid] = rowmaps[static_cast<size_t>(tid) * map_stride];
+    __syncthreads();
+    auto slot_of = [&](int row, bool insert) -> int {
+        unsigned h = (static_cast<unsigned>(row) * 2654435761u) >> 21;  // 11 bits
+        for (;;) {
+            const int k = hkey[h];
+            if (k == row) return h;
+            if (k == -1) {
+                if (!insert) return -1;
+                const int old = atomicCAS(&hkey[h], -1, row);
+                if (old == -1 || old == row) return h;
+            }
+            h = (h + 1) & (HS - 1);
+        }
+    };
+    // every row a map names: dst and src sets of one map are the same set
+    for (int idx = tid; idx < nmaps * 64; idx += 256) {
+        const int m = idx >> 6, q = idx & 63;
+        if (q < cnts[m] && q < W2) {
+            const int row = rowmaps[static_cast<size_t>(m) * map_stride + 1 + q];
+            const int h = slot_of(row, true);
+            (void)h;
+        }
+    }
+    __syncthreads();
+    for (int h = tid; h < HS; h += 256)
+        if (hkey[h] != -1) {
+            const int sl = atomicAdd(&nslots, 1);
+            hslot[h] = sl;
+            rowof[sl] = hkey[h];
+        }
+    __syncthreads();
+    for (int idx = tid; idx < nmaps * 64; idx += 256) {
+        const int m = idx >> 6, q = idx & 63;
+        if (q < cnts[m] && q < W2) {
+            const int* rm = rowmaps + static_cast<size_t>(m) * map_stride;
+            sdst[m][q] = static_cast<short>(hslot[slot_of(rm[1 + q], false)]);
+            ssrc[m][q] = static_cast<short>(hslot[slot_of(rm[1 + W2 + q], false)]);
+        }
+    }
+    const int ns = nslots;
+    for (int idx = tid; idx < ns * SC; idx += 256) {
+        const int sl = idx / SC, c = idx - sl * SC;
+        const long long t = s0 + c;
+        if (t < total) {
+            const long long col = (t < la) ? c0a + t : c0b + (t - la);
+            buf[sl][c] = A[static_cast<long long>(rowof[sl]) * lda + col];
+        }
+    }
+    __syncthreads();
+    for (int m = 0; m < nmaps; ++m) {
+        const int cnt = cnts[m];
+        T v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = tid + 256 * k;
+            const int q = idx / SC, c = idx - q * SC;
+            if (q < cnt) v[k] = buf[ssrc[m][q]][c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = tid + 256 * k;
+            const int q = idx / SC, c = idx - q * SC;
+            if (q < cnt) buf[sdst[m][q]][c] = v[k];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < ns * SC; idx += 256) {
+        const int sl = idx / SC, c = idx - sl * SC;
+        const long long t = s0 + c;
+        if (t < total) {
+            const long long col = (t < la) ? c0a + t : c0b + (t - la);
+            A[static_cast<long long>(rowof[sl]) * lda + col] = buf[sl][c];
+        }
+    }
+}
+
+// U12 = L11^-1 A12 for a whole block column in one launch: rows K..K+B-1 (B <= 256), columns
+// [c0, c1), L11 the unit-lower B x B block at (K, K). Each workgroup owns 64 columns and keeps
+// the finished U rows of its columns in LDS. The rows are done in 32-row groups: first every
+// row of the group takes the contributions of all earlier groups (one row and 8 columns per
+// thread), then one thread per column finishes the group by forward substitution against
+// the 32 x 32 diagonal block. Every entry receives its updates in increasing pivot order with
+// each product rounded before the subtraction, which is exactly the row-wise forward
+// substitution of src/matrix_operation/cpu/native/lu_decomposition_native.cpp:364-380, so the
+// result does not depend on the grouping.
+constexpr int kU12TN = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long long lda, long long K, int B,
+    long long c0, long long c1)
+{
+    constexpr int TN = kU12TN;
+    __shared__ T U[256][TN + 1];
+    __shared__ T Ld[32][33];
+    __shared__ T Lc[2][32][33];  // 32 x 32 pieces of the rows' L, double-buffered
+    const int tid = threadIdx.x;
+    const long long cb = c0 + static_cast<long long>(blockIdx.x) * TN;
+    const int ncols = static_cast<int>(min(static_cast<long long>(TN), c1 - cb));
+    const int nin = (B + 31) / 32;
+    const int r = tid >> 3, cg = (tid & 7) * 8;
+    const int lr = tid >> 3, lc = (tid & 7) * 4;  // staging: 4 consecutive L entries per thread
+    for (int p = 0; p < nin; ++p) {
+        const long long r0 = K + 32 * p;
+        const int rp = min(32, B - 32 * p);
+        T x[8];
+        if (r < rp) {
+            const T* arow = A + (r0 + r) * lda + cb + cg;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = (cg + e < ncols) ? arow[e] : T(0);
+        }
+        // contributions of the earlier groups, 32 pivots at a time
+        T st[4];
+        auto fetch = [&](int qc) {
+            const T* src = A + (r0 + lr) * lda + K + 32 * qc + lc;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) st[e] = (lr < rp) ? src[e] : T(0);
+        };
+        if (p > 0) fetch(0);
+        for (int qc = 0; qc < p; ++qc) {
+            const int buf = qc & 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Lc[buf][lr][lc + e] = st[e];
+            __syncthreads();
+            if (qc + 1 < p) fetch(qc + 1);
+            if (r < rp) {
+#pragma unroll 8
+                for (int qq = 0; qq < 32; ++qq) {
+                    const T l = Lc[buf][r][qq];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) x[e] = x[e] - l * U[32 * qc + qq][cg + e];
+                }
+            }
+        }
+        if (r < rp) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) U[32 * p + r][cg + e] = x[e];
+        }
+        for (int idx = tid; idx < 32 * 32; idx += 256) {
+            const int i = idx >> 5, q = idx & 31;
+            Ld[i][q] = (i < rp && q < rp) ? A[(r0 + i) * lda + r0 + q] : T(0);
+        }
+        __syncthreads();
+        if (tid < TN) {
+            // the column stays in LDS; the diagonal-block entries are broadcast reads
+            const int c = tid;
+            T* ucol = &U[32 * p][c];
+#pragma unroll 1
+            for (int i = 1; i < rp; ++i) {
+                T sum = ucol[i * (TN + 1)];
+#pragma unroll 8
+                for (int q = 0; q < i; ++q) sum = sum - Ld[i][q] * ucol[q * (TN + 1)];
+                ucol[i * (TN + 1)] = sum;
+            }
+            if (c < ncols)
+                for (int i = 0; i < rp; ++i) A[(r0 + i) * lda + cb + c] = ucol[i * (TN + 1)];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------- GEMM
 
 template <typename T> struct Mfma16;

@@ -46,7 +46,9 @@ __device__ __forceinline__ void st_chunk(Chunk* p, double v, int idx, unsigned t
     d.y = static_cast<unsigned>(bits >> 32);
     d.z = static_cast<unsigned>(idx);
     d.w = tag;
-    asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(d) : "memory");
+    // s_nop: a store of more than 8 bytes needs one wait state before a VALU instruction may
+    // overwrite its data registers, which the compiler does not insert after inline asm
+    asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 1" : : "v"(p), "v"(d) : "memory");
 }
 
 // N agent-coherent 16-byte loads in flight at once; the wait is inside the same asm
@@ -123,6 +125,46 @@ template <> __device__ __forceinline__ void ld_chunks<9>(u32x4 (&r)[9], const Ch
         : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7]), "=&v"(r[8])
         : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]), "v"(p[8])
         : "memory");
+}
+
+// Bounded poll of N chunks until all carry `tag` (raises *err and returns on expiry).
+template <int N>
+__device__ __forceinline__ void spin_chunks(u32x4 (&r)[N], const Chunk* const (&p)[N], unsigned tag, unsigned spin_limit,
+    int* err)
+{
+    unsigned spins = 0;
+    for (;;) {
+        ld_chunks<N>(r, p);
+        bool ok = true;
+#pragma unroll
+        for (int s = 0; s < N; ++s)
+            if (r[s].w != tag) ok = false;
+        if (ok) return;
+        ++spins;
+        if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ void spin_chunk1(u32x4& r, const Chunk* p, unsigned tag, unsigned spin_limit, int* err)
+{
+    unsigned spins = 0;
+    for (;;) {
+        const Chunk* const pc[1] = {p};
+        u32x4 rr[1];
+        ld_chunks<1>(rr, pc);
+        r = rr[0];
+        if (r.w == tag) return;
+        ++spins;
+        if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 __device__ __forceinline__ double chunk_value(const u32x4& c)
@@ -384,6 +426,304 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
         A[(row0 + r) * lda + k + c] = tile[r * LD + c];
     }
     if (b == 0 && tid < kWave) panel_rowmap(pivs, k, w, 2 * W, ws.rowmap);
+}
+
+// Register-resident version. Thread t of workgroup b holds RPT consecutive rows of the panel
+// (rows k + b*R + t*RPT + q, R = 256*RPT) with all W columns in registers, so lane order is
+// row order for the pivot search and the rank-1 update needs no LDS traffic. Per column:
+//   1. argmax over the thread's rows, DPP wave argmax, one barrier to combine the four waves;
+//   2. the thread holding the workgroup's candidate (and, in workgroup 0, the thread holding
+//      the diagonal row) copies the row to LDS, and 32 lanes of its wave publish it as tagged
+//      chunks with one store each;
+//   3. phase A: thread t reads workgroup t's candidate entry of the current column (one chunk,
+//      repeated until its tag matches); one barrier; every wave picks the winner itself;
+//   4. phase B: every workgroup reads the winner's row, and the winner's workgroup (which
+//      holds row p) the diagonal row; one barrier;
+//   5. the threads holding rows i and p exchange them; every thread forms its multipliers and
+//      updates its rows.
+// Three barriers per column and 96 chunks read per workgroup. The pivot rule, the
+// interchange, the multiplier (x * (1/d) unless |d| < T_MIN, then x / d) and the update order
+// (each product rounded before it is subtracted, columns updated left to right) are those of
+// src/matrix_operation/cpu/native/lu_decomposition_native.cpp:158-232.
+// With P == 1 the candidate and diagonal rows go through LDS instead of memory.
+
+// State of one thread of getf2_panel4_kernel; step<J>() is column J, with J a compile-time
+// constant so every access to the row registers is static.
+// Copies row q of the thread's registers to an LDS row (as doubles).
+template <typename T, int W, int RPT>
+__device__ __forceinline__ void stage_row(const T (&v)[RPT][W], int q, double* dst)
+{
+#pragma unroll
+    for (int qq = 0; qq < RPT; ++qq)
+        if (qq == q) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) dst[c] = static_cast<double>(v[qq][c]);
+        }
+}
+
+template <typename T, int W, int RPT> struct Panel4 {
+    T v[RPT][W];
+    double (*stage)[2][W];
+    T* A;
+    long long lda, k, n, my0, row0;
+    int w, P, tid, lane, wid, b;
+    unsigned tag_base, spin_limit;
+    long long* piv;
+    Panel2Ws<T> ws;
+    double* red_v;
+    long long* red_i;
+    double (*cand)[kPanel2MaxWG][W];
+    int (*cand_idx)[kPanel2MaxWG];
+    double (*drow)[W];
+    long long* pivs;
+
+    template <int J> __device__ __forceinline__ void step()
+    {
+        constexpr int R = kPanelThreads * RPT;
+        const long long i = k + J;
+        constexpr int par = J & 1;
+        const unsigned tag = tag_base + static_cast<unsigned>(J) + 1u;
+
+        // 1. workgroup candidate
+        double bv = -2.0;
+        long long bi = 0x7fffffffffffffffll;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const long long g = my0 + q;
+            if (g >= i && g < n) {
+                const double key = pivot_key(v[q][J], g == i);
+                if (key > bv) {
+                    bv = key;
+                    bi = g;
+                }
+            }
+        }
+        wave_argmax_ordered(bv, bi);
+        if (lane == 0) {
+            red_v[wid] = bv;
+            red_i[wid] = bi;
+        }
+        __syncthreads();
+        bv = red_v[0];
+        bi = red_i[0];
+#pragma unroll
+        for (int q = 1; q < kPanelThreads / kWave; ++q)
+            if (argmax_better(red_v[q], red_i[q], bv, bi)) {
+                bv = red_v[q];
+                bi = red_i[q];
+            }
+
+        // 2. publish (P > 1) or hand over through LDS (P == 1): the owner lane copies the row to
+        //    its wave's staging area, then lane c of that wave handles column c
+        const bool own_cand = bv >= -1.0 && bi >= my0 && bi < my0 + RPT;
+        const bool own_diag = i >= my0 && i < my0 + RPT;
+        const bool wave_cand = bv >= -1.0 && bi >= row0 && ((bi - row0) / RPT) / kWave == wid;
+        const bool wave_diag = i >= row0 && i < row0 + R && ((i - row0) / RPT) / kWave == wid;
+        if (own_cand) stage_row(v, static_cast<int>(bi - my0), stage[wid][0]);
+        if (own_diag) stage_row(v, static_cast<int>(i - my0), stage[wid][1]);
+        if (wave_cand || wave_diag) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < w) {
+                if (wave_cand) {
+                    if (P > 1)
+                        st_chunk(&ws.slots[(static_cast<size_t>(par) * kPanel2MaxWG + b) * W + lane], stage[wid][0][lane],
+                            static_cast<int>(bi), tag);
+                    else
+                        cand[par][0][lane] = stage[wid][0][lane];
+                }
+                if (wave_diag) {
+                    if (P > 1)
+                        st_chunk(&ws.dslots[par * W + lane], stage[wid][1][lane], static_cast<int>(i), tag);
+                    else
+                        drow[par][lane] = stage[wid][1][lane];
+                }
+            }
+            if (P == 1 && wave_cand && lane == 0) cand_idx[par][0] = static_cast<int>(bi);
+        }
+
+        // 3. phase A: every workgroup's candidate entry of column J (thread t reads workgroup t's)
+        if (P > 1) {
+            if (tid < P) {
+                u32x4 r;
+                spin_chunk1(r, &ws.slots[(static_cast<size_t>(par) * kPanel2MaxWG + tid) * W + J], tag, spin_limit,
+                    ws.err);
+                cand[par][tid][J] = chunk_value(r);
+                cand_idx[par][tid] = static_cast<int>(r.z);
+            }
+            __syncthreads();
+        } else {
+            __syncthreads();
+        }
+
+        // 4. winner (each wave on its own)
+        double gv = -2.0;
+        long long gi = 0x7fffffffffffffffll;
+        if (lane < P) {
+            gi = cand_idx[par][lane];
+            gv = pivot_key(static_cast<T>(cand[par][lane][J]), gi == i);
+        }
+        wave_argmax_ordered(gv, gi);  // lane b holds workgroup b's row range
+        long long p = gi;
+        if (p < i || p >= n) {
+            // only a failed exchange can get here: report it and leave every row in place
+            if (tid == 0) __hip_atomic_store(ws.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p = i;
+        }
+        const int wb = static_cast<int>(min(max((p - k) / R, 0ll), static_cast<long long>(P - 1)));
+
+        // 5. phase B: the winner's row (every workgroup) and the diagonal row (the winner's
+        //    workgroup, which holds row p)
+        if (P > 1) {
+            if (tid < w) {
+                u32x4 r;
+                spin_chunk1(r, &ws.slots[(static_cast<size_t>(par) * kPanel2MaxWG + wb) * W + tid], tag, spin_limit,
+                    ws.err);
+                cand[par][wb][tid] = chunk_value(r);
+            } else if (b == wb && tid >= kWave && tid - kWave < w) {
+                u32x4 r;
+                spin_chunk1(r, &ws.dslots[par * W + (tid - kWave)], tag, spin_limit, ws.err);
+                drow[par][tid - kWave] = chunk_value(r);
+            }
+            __syncthreads();
+        }
+        const double* prow = cand[par][wb];
+
+        // 6. interchange, multipliers, rank-1 update
+        if (p != i) {
+            if (own_diag) {
+                const int qd = static_cast<int>(i - my0);
+#pragma unroll
+                for (int q = 0; q < RPT; ++q)
+                    if (q == qd) {
+#pragma unroll
+                        for (int c = 0; c < W; ++c)
+                            if (c < w) v[q][c] = static_cast<T>(prow[c]);
+                    }
+            }
+            if (p >= my0 && p < my0 + RPT) {
+                const int qp = static_cast<int>(p - my0);
+#pragma unroll
+                for (int q = 0; q < RPT; ++q)
+                    if (q == qp) {
+#pragma unroll
+                        for (int c = 0; c < W; ++c)
+                            if (c < w) v[q][c] = static_cast<T>(drow[par][c]);
+                    }
+            }
+        }
+        if (b == 0 && tid == 0) {
+            piv[i] = p + 1;
+            pivs[J] = p;
+        }
+        const T d = static_cast<T>(prow[J]);
+        if (d == T(0)) {
+            if (b == 0 && tid == 0 && *ws.info == 0) *ws.info = static_cast<int>(i + 1);
+        } else {
+            const bool use_recip = fabs(static_cast<double>(d)) >= static_cast<double>(TypeMin<T>::v);
+            const T rcp = use_recip ? T(1) / d : T(1);
+            T l[RPT];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const long long g = my0 + q;
+                l[q] = T(0);
+                if (g > i && g < n) {
+                    l[q] = use_recip ? v[q][J] * rcp : v[q][J] / d;
+                    v[q][J] = l[q];
+                }
+            }
+            // columns in groups of 8, so only eight pivot-row values are live at a time
+#pragma unroll
+            for (int c0 = (J + 1) / 8 * 8; c0 < W; c0 += 8) {
+#pragma unroll
+                for (int c = c0; c < c0 + 8; ++c) {
+                    if (c > J) {
+                        const T u = static_cast<T>(prow[c]);
+#pragma unroll
+                        for (int q = 0; q < RPT; ++q)
+                            if (my0 + q > i && my0 + q < n) v[q][c] = v[q][c] - l[q] * u;
+                    }
+                }
+                asm volatile("" ::: "memory");
+            }
+        }
+    }
+
+    template <int J> __device__ __forceinline__ void run()
+    {
+        if constexpr (J < W) {
+            if (J < w) {
+                step<J>();
+                run<J + 1>();
+            }
+        }
+    }
+};
+
+// Register-resident kernel: see the comment above stage_row / Panel4.
+template <typename T, int W, int RPT>
+__global__ __launch_bounds__(kPanelThreads) void getf2_panel4_kernel(T* __restrict__ A, long long lda, long long k,
+    long long m, int w, int P, unsigned tag_base, unsigned spin_limit, long long* __restrict__ piv, Panel2Ws<T> ws)
+{
+    constexpr int R = kPanelThreads * RPT;
+    __shared__ double red_v[kPanelThreads / kWave];
+    __shared__ long long red_i[kPanelThreads / kWave];
+    __shared__ double cand[2][kPanel2MaxWG][W];
+    __shared__ int cand_idx[2][kPanel2MaxWG];
+    __shared__ double drow[2][W];
+    __shared__ long long pivs[W];
+    __shared__ double stage[kPanelThreads / kWave][2][W];
+
+    Panel4<T, W, RPT> st;
+    st.A = A;
+    st.lda = lda;
+    st.k = k;
+    st.n = k + m;
+    st.w = w;
+    st.P = P;
+    st.tid = threadIdx.x;
+    st.lane = st.tid & (kWave - 1);
+    st.wid = st.tid / kWave;
+    st.b = blockIdx.x;
+    st.row0 = k + static_cast<long long>(st.b) * R;
+    st.my0 = st.row0 + static_cast<long long>(st.tid) * RPT;
+    st.tag_base = tag_base;
+    st.spin_limit = spin_limit;
+    st.piv = piv;
+    st.ws = ws;
+    st.red_v = red_v;
+    st.red_i = red_i;
+    st.cand = cand;
+    st.cand_idx = cand_idx;
+    st.drow = drow;
+    st.pivs = pivs;
+    st.stage = stage;
+
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const long long g = st.my0 + q;
+        const T* src = A + (g < st.n ? g : k) * lda + k;
+#pragma unroll
+        for (int c = 0; c < W; ++c) st.v[q][c] = (g < st.n && c < w) ? src[c] : T(0);
+    }
+
+    st.template run<0>();
+
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const long long g = st.my0 + q;
+        if (g < st.n) {
+            T* dst = A + g * lda + k;
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+                if (c < w) dst[c] = st.v[q][c];
+        }
+    }
+    if (st.b == 0) {
+        __syncthreads();
+        if (st.tid < kWave) panel_rowmap(pivs, k, w, 2 * W, ws.rowmap);
+    }
 }
 
 }  // namespace lrn
