@@ -64,8 +64,10 @@ _RESTYPES = {"lrn_status_string": ctypes.c_char_p, "lrn_version": ctypes.c_char_
 _lib = None
 
 
-def load():
-    """Load (once) and return the library; raises if it has not been built."""
+def load(apply_tuning: bool = True):
+    """Load (once) and return the library; raises if it has not been built. On the first load
+    the block sizes of ``tuning.json`` (written by ``python -m linearalgebra_amd.autotune``)
+    are applied, if that file exists and ``apply_tuning`` is set."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
@@ -78,6 +80,9 @@ def load():
             fn.argtypes = args
             fn.restype = _RESTYPES.get(name, _int)
         _lib = lib
+        if apply_tuning:
+            from . import autotune
+            autotune.apply(lib)
     return _lib
 
 

@@ -1,0 +1,143 @@
+"""Autotuner of the run-time block sizes (writes ``tuning.json`` next to this file).
+
+    python -m linearalgebra_amd.autotune [--n 16384] [--batch-count 10000] [--out PATH]
+
+On the GPU it is run on, it times
+  * the FP64 blocked LU at order ``--n`` for every outer block width (128, 256, 384, 512),
+    inner panel width (32, 64) and 32-column panel kernel (register-resident rows or rows in
+    LDS) -- the settings of ``lrn_set_tuning`` / ``lrn_set_panel_kernel``;
+  * the batched small-system solver on the uneven 10 000-system batch (n ~ U[8, 256],
+    nrhs ~ U[1, 8]) for persistent grids of 1, 2 and 3 workgroups per compute unit and the
+    occupancy-derived default -- the setting of ``lrn_set_batched_grid``;
+and writes the fastest of each, with every measurement, to ``tuning.json``. The package
+applies that file when the library is loaded (``_lib.load``); the C ABI setters take the
+same values for callers that do not use the Python package.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_PATH = os.path.join(HERE, "tuning.json")
+
+
+def _time_getrf(n: int, outer_nb: int, panel_w: int, panel_kernel: int, reps: int = 2) -> float:
+    import linearalgebra_amd as la
+    lib = _lib.load()
+    if lib.lrn_set_tuning(outer_nb, panel_w, 0) != 0 or lib.lrn_set_panel_kernel(panel_kernel) != 0:
+        return float("inf")
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    w = torch.empty_like(a)
+    piv = torch.empty(n, dtype=torch.int64, device=dev)
+    ws = torch.empty(la.lu_decomposition.workspace_size(n), dtype=torch.uint8, device=dev)
+    best = float("inf")
+    for _ in range(reps):
+        w.copy_(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            la.lu_decomposition.apply(w, n, piv, ws)
+        except Exception:  # an unsupported combination (e.g. too many workgroups)
+            return float("inf")
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def _time_batched(count: int, grid: int, reps: int = 5) -> float:
+    from .batched import BatchedSystems
+    lib = _lib.load()
+    if lib.lrn_set_batched_grid(grid) != 0:
+        return float("inf")
+    rng = np.random.default_rng(5)
+    n = rng.integers(8, 257, size=count)
+    nrhs = rng.integers(1, 9, size=count)
+    bs = BatchedSystems(n, nrhs, torch.float64, device="cuda:0")
+    g = torch.Generator(device=bs.a.device)
+    g.manual_seed(3)
+    bs.a.normal_(generator=g)
+    bs.b.normal_(generator=g)
+    a0, b0 = bs.a.clone(), bs.b.clone()
+    best = float("inf")
+    for _ in range(reps):
+        bs.a.copy_(a0)
+        bs.b.copy_(b0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs.solve()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune(n: int = 16384, batch_count: int = 10000, out: str = TUNING_PATH) -> dict:
+    lib = _lib.load(apply_tuning=False)
+    props = torch.cuda.get_device_properties(0)
+    cus = props.multi_processor_count
+    getrf = []
+    for outer_nb in (128, 256, 384, 512):
+        for panel_w in (32, 64):
+            for panel_kernel in ((0, 1) if panel_w == 32 else (0,)):
+                t = _time_getrf(n, outer_nb, panel_w, panel_kernel)
+                getrf.append({"outer_nb": outer_nb, "panel_w": panel_w, "panel_kernel": panel_kernel,
+                              "ms": round(1e3 * t, 2) if t != float("inf") else None})
+                print(f"getrf n={n} outer_nb={outer_nb} panel_w={panel_w} panel_kernel={panel_kernel}: "
+                      f"{getrf[-1]['ms']} ms", flush=True)
+    lib.lrn_set_tuning(0, 0, 0)
+    lib.lrn_set_panel_kernel(0)
+    batched = []
+    for grid in (0, cus, 2 * cus, 3 * cus):
+        t = _time_batched(batch_count, grid)
+        batched.append({"grid": grid, "ms": round(1e3 * t, 3) if t != float("inf") else None})
+        print(f"batched count={batch_count} grid={grid or 'occupancy'}: {batched[-1]['ms']} ms", flush=True)
+    lib.lrn_set_batched_grid(0)
+    best_g = min((r for r in getrf if r["ms"] is not None), key=lambda r: r["ms"])
+    best_b = min((r for r in batched if r["ms"] is not None), key=lambda r: r["ms"])
+    result = {
+        "device": props.name, "compute_units": cus,
+        "getrf": {"outer_nb": best_g["outer_nb"], "panel_w": best_g["panel_w"], "panel_kernel": best_g["panel_kernel"],
+                  "order": n, "ms": best_g["ms"]},
+        "batched": {"grid": best_b["grid"], "count": batch_count, "ms": best_b["ms"]},
+        "measurements": {"getrf": getrf, "batched": batched},
+    }
+    with open(out, "w") as f:
+        json.dump(result, f, indent=1)
+    print(f"wrote {out}: getrf {result['getrf']}, batched {result['batched']}", flush=True)
+    return result
+
+
+def apply(lib, path: str = TUNING_PATH) -> bool:
+    """Apply a tuning file to the loaded library; False if there is none."""
+    if not os.path.exists(path):
+        return False
+    with open(path) as f:
+        t = json.load(f)
+    g = t.get("getrf", {})
+    lib.lrn_set_tuning(int(g.get("outer_nb", 0)), int(g.get("panel_w", 0)), 0)
+    lib.lrn_set_panel_kernel(int(g.get("panel_kernel", 0)))
+    lib.lrn_set_batched_grid(int(t.get("batched", {}).get("grid", 0)))
+    return True
+
+
+def main() -> None:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--batch-count", type=int, default=10000)
+    p.add_argument("--out", default=TUNING_PATH)
+    a = p.parse_args()
+    tune(a.n, a.batch_count, a.out)
+
+
+if __name__ == "__main__":
+    main()
