@@ -3,9 +3,10 @@
     python -m linearalgebra_amd.autotune [--n 16384] [--batch-count 10000] [--out PATH]
 
 On the GPU it is run on, it times
-  * the FP64 blocked LU at order ``--n`` for every outer block width (128, 256, 384, 512),
-    inner panel width (32, 64) and 32-column panel kernel (register-resident rows or rows in
-    LDS) -- the settings of ``lrn_set_tuning`` / ``lrn_set_panel_kernel``;
+  * the FP64 and FP32 blocked LU at order ``--n`` for every outer block width (128, 256, 384,
+    512), inner panel width (32, 64) and 32-column panel kernel (register-resident rows or
+    rows in LDS) -- the settings of ``lrn_set_tuning`` / ``lrn_set_panel_kernel``, which are
+    process-wide, so the fastest sum of the two is kept;
   * the batched small-system solver on the uneven 10 000-system batch (n ~ U[8, 256],
     nrhs ~ U[1, 8]) for persistent grids of 1, 2 and 3 workgroups per compute unit and the
     occupancy-derived default -- the setting of ``lrn_set_batched_grid``;
@@ -29,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.path.join(HERE, "tuning.json")
 
 
-def _time_getrf(n: int, outer_nb: int, panel_w: int, panel_kernel: int, reps: int = 2) -> float:
+def _time_getrf(n: int, outer_nb: int, panel_w: int, panel_kernel: int, dtype=torch.float64, reps: int = 2) -> float:
     import linearalgebra_amd as la
     lib = _lib.load()
     if lib.lrn_set_tuning(outer_nb, panel_w, 0) != 0 or lib.lrn_set_panel_kernel(panel_kernel) != 0:
@@ -37,10 +38,10 @@ def _time_getrf(n: int, outer_nb: int, panel_w: int, panel_kernel: int, reps: in
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(11)
-    a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    a = torch.randn(n, n, dtype=dtype, device=dev, generator=g)
     w = torch.empty_like(a)
     piv = torch.empty(n, dtype=torch.int64, device=dev)
-    ws = torch.empty(la.lu_decomposition.workspace_size(n), dtype=torch.uint8, device=dev)
+    ws = torch.empty(la.lu_decomposition.workspace_size(n, dtype), dtype=torch.uint8, device=dev)
     best = float("inf")
     for _ in range(reps):
         w.copy_(a)
@@ -89,11 +90,16 @@ def tune(n: int = 16384, batch_count: int = 10000, out: str = TUNING_PATH) -> di
     for outer_nb in (128, 256, 384, 512):
         for panel_w in (32, 64):
             for panel_kernel in ((0, 1) if panel_w == 32 else (0,)):
-                t = _time_getrf(n, outer_nb, panel_w, panel_kernel)
+                # the setting is process-wide: the plain FP64 and the mixed solver's FP32
+                # factorisation both count
+                t64 = _time_getrf(n, outer_nb, panel_w, panel_kernel, torch.float64)
+                t32 = _time_getrf(n, outer_nb, panel_w, panel_kernel, torch.float32)
+                ok = t64 != float("inf") and t32 != float("inf")
                 getrf.append({"outer_nb": outer_nb, "panel_w": panel_w, "panel_kernel": panel_kernel,
-                              "ms": round(1e3 * t, 2) if t != float("inf") else None})
+                              "ms_f64": round(1e3 * t64, 2) if ok else None, "ms_f32": round(1e3 * t32, 2) if ok else None,
+                              "ms": round(1e3 * (t64 + t32), 2) if ok else None})
                 print(f"getrf n={n} outer_nb={outer_nb} panel_w={panel_w} panel_kernel={panel_kernel}: "
-                      f"{getrf[-1]['ms']} ms", flush=True)
+                      f"f64 {getrf[-1]['ms_f64']} ms, f32 {getrf[-1]['ms_f32']} ms", flush=True)
     lib.lrn_set_tuning(0, 0, 0)
     lib.lrn_set_panel_kernel(0)
     batched = []
@@ -107,7 +113,7 @@ def tune(n: int = 16384, batch_count: int = 10000, out: str = TUNING_PATH) -> di
     result = {
         "device": props.name, "compute_units": cus,
         "getrf": {"outer_nb": best_g["outer_nb"], "panel_w": best_g["panel_w"], "panel_kernel": best_g["panel_kernel"],
-                  "order": n, "ms": best_g["ms"]},
+                  "order": n, "ms_f64": best_g["ms_f64"], "ms_f32": best_g["ms_f32"]},
         "batched": {"grid": best_b["grid"], "count": batch_count, "ms": best_b["ms"]},
         "measurements": {"getrf": getrf, "batched": batched},
     }

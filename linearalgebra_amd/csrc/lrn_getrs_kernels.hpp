@@ -133,6 +133,9 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
         }
     }
     for (int idx = tid; idx < TB * kSolveMaxCols; idx += 256) Acc[idx / kSolveMaxCols][idx % kSolveMaxCols] = T(0);
+    // Acc is cleared with a different thread-to-entry mapping than the one that reads it
+    // below; with no earlier block (jcount == 0) no other barrier separates the two
+    __syncthreads();
 
     // off-diagonal contributions in the order the producing blocks complete: increasing j
     // for the forward pass, decreasing j for the backward pass (block nblk-1 first), so a

@@ -116,3 +116,24 @@ def test_cxx_backend_host_checks():
     out = subprocess.run([exe, "--no-gpu"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "all checks passed" in out.stdout
+
+
+def test_tuning_file_is_applied(tmp_path):
+    """autotune.apply sets the process-wide block sizes from a tuning file (no GPU needed)."""
+    import ctypes
+    import json
+    from linearalgebra_amd import _lib, autotune
+    lib = _lib.load()
+    path = tmp_path / "tuning.json"
+    path.write_text(json.dumps({"getrf": {"outer_nb": 384, "panel_w": 32, "panel_kernel": 1},
+                                "batched": {"grid": 0}}))
+    try:
+        assert autotune.apply(lib, str(path))
+        nb, w, mx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.lrn_get_tuning(ctypes.byref(nb), ctypes.byref(w), ctypes.byref(mx))
+        assert nb.value == 384 and w.value == 32
+    finally:
+        lib.lrn_set_tuning(0, 0, 0)
+        lib.lrn_set_panel_kernel(0)
+        autotune.apply(lib)
+    assert not autotune.apply(lib, str(tmp_path / "absent.json"))

@@ -160,3 +160,51 @@ def test_getrf_f32_matches_oracle(n, dev):
     mis = O.first_pivot_mismatch(piv_g, piv_o)
     assert mis == -1 or mis > n // 2
     assert O.reconstruction_error(a, lu_g, piv_g) < 1.0
+
+
+@pytest.mark.parametrize("n", [300, 1000, 4100])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_panel_kernels_agree(n, dtype, dev):
+    """The register-resident and the LDS-resident 32-column panel kernels implement the same
+    elimination order, so pivots and factors are bitwise equal; both match the oracle."""
+    import torch
+    import linearalgebra_amd as la
+    from linearalgebra_amd import _lib
+    lib = _lib.load()
+    a = O.gaussian(n, seed=900 + n)
+    if dtype == "f32":
+        a = a.astype(np.float32)
+    out = {}
+    try:
+        for kind in (0, 1):
+            assert lib.lrn_set_panel_kernel(kind) == 0
+            out[kind] = _gpu_getrf(a, dev)
+    finally:
+        lib.lrn_set_panel_kernel(0)
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    lu_o, piv_o, _ = O.getrf(a)
+    mis = O.first_pivot_mismatch(out[0][1], piv_o)
+    print(f"panel kernels n={n} {dtype}: first pivot mismatch vs oracle {mis}", flush=True)
+    if n <= 1000:
+        assert mis == -1
+
+
+@pytest.mark.parametrize("outer_nb", [128, 384, 512])
+def test_outer_block_widths_match_oracle(outer_nb, dev):
+    """Block widths the autotuner may pick (fused block-column kernels and their fallbacks)."""
+    from linearalgebra_amd import _lib
+    lib = _lib.load()
+    n = 1100
+    a = O.gaussian(n, seed=77)
+    try:
+        assert lib.lrn_set_tuning(outer_nb, 32, 0) == 0
+        lu_g, piv_g, info_g, ok = _gpu_getrf(a, dev)
+    finally:
+        lib.lrn_set_tuning(0, 0, 0)
+        from linearalgebra_amd import autotune
+        autotune.apply(lib)
+    lu_o, piv_o, info_o = O.getrf(a)
+    assert info_g == info_o == 0 and ok
+    assert np.array_equal(piv_g, piv_o), f"first mismatch at {O.first_pivot_mismatch(piv_g, piv_o)}"
+    assert O.rel_diff(lu_g, lu_o) <= 1e-12
