@@ -34,8 +34,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 METRIC = "FP64-accurate Ax=b solves/s: mixed LU-IR vs plain FP64 LU, n=16384; batch solves/s"
 PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 TRACE = {"gemm_f64": 0, "gemm_f64_edge": 1, "gemm_bf16x3": 2, "gemm_bf16": 3, "gemm_f32": 4, "panel": 5,
-         "laswp": 6, "trsm": 7, "trsv": 8, "gemm_bf16x6": 9}
-FLOP_KINDS = {"gemm_f64", "gemm_f64_edge", "gemm_bf16x3", "gemm_bf16", "gemm_f32", "panel", "gemm_bf16x6"}
+         "laswp": 6, "trsm": 7, "trsv": 8, "gemm_bf16x6": 9, "block_column_update": 11}
+FLOP_KINDS = {"gemm_f64", "gemm_f64_edge", "gemm_bf16x3", "gemm_bf16", "gemm_f32", "panel", "gemm_bf16x6",
+              "block_column_update"}
 KERNEL_SYMBOL = {"gemm_f64": "gemm_sub_f64_kernel<false>", "gemm_bf16x3": "gemm_sub_bf16_kernel<2>",
                  "gemm_bf16x6": "gemm_sub_bf16_kernel<3>", "gemm_bf16": "gemm_sub_bf16_kernel<1>"}
 # bf16 matrix-core products per algorithmic product, by trailing-update mode

@@ -201,10 +201,15 @@ LRN_API int lrn_gesv_mixed_batched_f64(const double* a, const int64_t* a_offset,
  * value <= 0 restores the built-in choice. Not thread-safe against concurrent factorisations. */
 LRN_API int lrn_set_tuning(int outer_nb, int panel_w, int panel_max_wg);
 LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg);
-/* Look-ahead (next block column factored on a helper stream beside the trailing update):
- * on (1) or off (0, the default: with the current kernels the two streams do not overlap on
- * gfx950 and the split adds launches). Process-wide. */
-LRN_API int lrn_set_lookahead(int enable);
+/* Look-ahead: the next block column is updated and factored on helper streams while the
+ * rest of the trailing update runs (results are bitwise those without it). mode 0 = off,
+ * 1 = on with 32 compute units for the panel stream, a multiple of 8 (>= 8) = on with that
+ * many; orders above 16384 use at least 64. Process-wide; the autotuner picks the value. */
+LRN_API int lrn_set_lookahead(int mode);
+/* Update of the rest of a block column after each of its inner panels: 0 = 64 x 64 tiles on
+ * the vector units (the default), 1 = the matrix-core GEMM. Results differ only in rounding
+ * of the update's sums. Process-wide; the autotuner picks the value. */
+LRN_API int lrn_set_block_update(int kind);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,

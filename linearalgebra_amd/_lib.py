@@ -45,6 +45,7 @@ SIGNATURES = {
     "lrn_debug_panel": [_int, _vp, _i64, _i64, _int, _vp, _vp, _sz, _vp],
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
+    "lrn_set_block_update": [_int],
     "lrn_set_mixed_update": [_int],
     "lrn_gesv_mixed_batched_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, ctypes.c_double, _int, _int, _vp, _sz,
                                    _vp, _vp],

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -232,8 +233,14 @@ double panel_flops(long long m, int w)
 
 template <typename T, int W>
 int launch_panel(T* A, long long lda, long long k, long long m, int w, long long* piv, const PanelWs<T>& ws,
-    unsigned& base, hipStream_t s)
+    unsigned& base, hipStream_t s, int* need_cus = nullptr)
 {
+    // need_cus != nullptr: launch nothing, return the compute units the panel's cooperative
+    // workgroups need to be resident at once (workgroups / workgroups per unit)
+    auto need = [&](int P, int per_cu) {
+        *need_cus = (P > 1 && per_cu > 0) ? (P + per_cu - 1) / per_cu : 1;
+        return LRN_OK;
+    };
     int dev = 0;
     LRN_HIP(hipGetDevice(&dev));
     const DeviceProps& props = device_props(dev);
@@ -252,6 +259,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
             auto kern = getf2_panel4_kernel<T, W, 1>;
             int per_cu = 0;
             LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), 0, kPanelThreads, &per_cu));
+            if (need_cus != nullptr) return need(P, per_cu);
             if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
             TraceScope ts(kTrPanel, panel_flops(m, w), s);
             hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(P)), dim3(kPanelThreads), 0, s, A, lda, k, m, w, P, base,
@@ -283,6 +291,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
             auto go = [&](auto kern) -> int {
                 int per_cu = 0;
                 LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
+                if (need_cus != nullptr) return need(P, per_cu);
                 if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
                 TraceScope ts(kTrPanel, panel_flops(m, w), s);
                 hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit,
@@ -320,6 +329,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
     {
         int per_cu = 0;
         LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), lds, kPanelThreads, &per_cu));
+        if (need_cus != nullptr) return need(P, per_cu);
         if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
     }
     TraceScope ts(kTrPanel, panel_flops(m, w), s);
@@ -416,7 +426,7 @@ int launch_trsm(T* A, long long lda, long long k, int w, long long c0, long long
 // C[M x N] -= A[M x K] * B[K x N]
 template <typename T>
 int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M,
-    long long N, long long K, hipStream_t s)
+    long long N, long long K, hipStream_t s, int cus = 0)
 {
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
     if constexpr (std::is_same<T, double>::value) {
@@ -433,7 +443,7 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             // persistent grid: two workgroups per compute unit at most
             int dev = 0;
             LRN_HIP(hipGetDevice(&dev));
-            const long long cap = 2ll * device_props(dev).cus;
+            const long long cap = 2ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
             TraceScope ts(check ? kTrGemmF64Edge : kTrGemmF64, 2.0 * m * nn * K, s);
             if (check)
@@ -464,7 +474,7 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
             int dev = 0;
             LRN_HIP(hipGetDevice(&dev));
-            const long long cap = 2ll * device_props(dev).cus;
+            const long long cap = 2ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
             TraceScope ts(kTrGemmF32, 2.0 * m * nn * K, s);
             if (check)
@@ -497,7 +507,7 @@ std::atomic<int> g_mixed_panel{0};
 
 // C[M x N] -= A * B, FP32 data, with `planes` bf16 planes per operand (1..3)
 int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long long ldb, float* C, long long ldc,
-    long long M, long long N, long long K, hipStream_t s, int planes)
+    long long M, long long N, long long K, hipStream_t s, int planes, int cus = 0)
 {
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
     if (planes < 1 || planes > 3) return LRN_E_INTERNAL;
@@ -506,7 +516,7 @@ int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long lon
     const long long per_xcd = (tm * tn + 7) / 8;
     int dev = 0;
     LRN_HIP(hipGetDevice(&dev));
-    const long long cap = 4ll * device_props(dev).cus;
+    const long long cap = 4ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
     const dim3 grid(static_cast<unsigned>(std::min(per_xcd * 8, cap)));
     const int kind = planes == 1 ? kTrGemmBf16 : (planes == 2 ? kTrGemmBf16x3 : kTrGemmBf16x6);
     TraceScope ts(kind, 2.0 * M * N * K, s);
@@ -527,13 +537,13 @@ int launch_gemm_sub_bf16(const float* A, long long lda, const float* B, long lon
 // upd < 0 or 3: the GEMM of the data type.
 template <typename T>
 int launch_trailing(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M, long long N,
-    long long K, int upd, hipStream_t s)
+    long long K, int upd, hipStream_t s, int cus = 0)
 {
     if constexpr (std::is_same<T, float>::value) {
-        if (upd >= 0 && upd <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, upd + 1);
+        if (upd >= 0 && upd <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, upd + 1, cus);
     }
     (void)upd;
-    return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s);
+    return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s, cus);
 }
 
 int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
@@ -557,23 +567,33 @@ template <typename T> int reset_panel_state(const PanelWs<T>& ws, hipStream_t s)
     return LRN_OK;
 }
 
-// Per-device helper streams for the look-ahead: the block-column factorisation runs on a
-// stream restricted to the first `np` compute units while the rest of the trailing update
-// runs on a stream restricted to the others, so the panel's workgroups always find room.
-// Default priority, non-blocking.
+// Per-device helper streams for the look-ahead. Every launch of a look-ahead factorisation
+// goes to these streams (the caller's stream only hands over and takes back through events):
+// the legacy NULL stream, which callers commonly pass, waits for all work on blocking streams
+// before each of its launches, so work left on it would serialise the three streams.
+//   narrow: the first block column and each next block column's interchanges, U12 rows and
+//           update (the critical path); non-blocking, all compute units;
+//   panel : the block-column factorisations, restricted to `np` compute units so the
+//           cooperative panel's workgroups always find room;
+//   gemm  : the rest of each trailing update, restricted to the other compute units.
+// Default priority. `mu` keeps one factorisation at a time enqueuing on a set, so the
+// events below always refer to the same call.
 struct AsideStream {
     int np = 0;
+    hipStream_t narrow = nullptr;
     hipStream_t panel = nullptr;
     hipStream_t gemm = nullptr;
-    hipEvent_t ready = nullptr;      // recorded on the caller's stream: next block column updated
+    hipEvent_t start = nullptr;      // recorded on the caller's stream at entry
+    hipEvent_t ready = nullptr;      // recorded on the narrow stream: next block column updated
     hipEvent_t done = nullptr;       // recorded on the panel stream: next block column factored
     hipEvent_t rest_done = nullptr;  // recorded on the GEMM stream: rest of the trailing update done
+    std::mutex mu;
 };
 
 int aside_stream(int np, AsideStream** out)
 {
     static std::mutex mu;
-    static std::vector<std::vector<AsideStream>> per_dev;
+    static std::vector<std::vector<std::unique_ptr<AsideStream>>> per_dev;
     int dev = 0;
     LRN_HIP(hipGetDevice(&dev));
     const int cus = device_props(dev).cus;
@@ -581,28 +601,31 @@ int aside_stream(int np, AsideStream** out)
     std::lock_guard<std::mutex> g(mu);
     if (dev >= static_cast<int>(per_dev.size())) per_dev.resize(dev + 1);
     for (auto& a : per_dev[dev])
-        if (a.np == np) {
-            *out = &a;
+        if (a->np == np) {
+            *out = a.get();
             return LRN_OK;
         }
-    AsideStream a;
-    a.np = np;
+    auto a = std::make_unique<AsideStream>();
+    a->np = np;
     const int words = (cus + 31) / 32;
     std::vector<uint32_t> mp(words, 0u), mg(words, 0u);
     for (int c = 0; c < cus; ++c) (c < np ? mp : mg)[c / 32] |= 1u << (c % 32);
-    LRN_HIP(hipExtStreamCreateWithCUMask(&a.panel, static_cast<uint32_t>(words), mp.data()));
-    LRN_HIP(hipExtStreamCreateWithCUMask(&a.gemm, static_cast<uint32_t>(words), mg.data()));
-    LRN_HIP(hipEventCreateWithFlags(&a.ready, hipEventDisableTiming));
-    LRN_HIP(hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
-    LRN_HIP(hipEventCreateWithFlags(&a.rest_done, hipEventDisableTiming));
-    per_dev[dev].reserve(4);
-    if (per_dev[dev].size() >= 4) return LRN_E_INTERNAL;
-    per_dev[dev].push_back(a);
-    *out = &per_dev[dev].back();
+    LRN_HIP(hipStreamCreateWithFlags(&a->narrow, hipStreamNonBlocking));
+    LRN_HIP(hipExtStreamCreateWithCUMask(&a->panel, static_cast<uint32_t>(words), mp.data()));
+    LRN_HIP(hipExtStreamCreateWithCUMask(&a->gemm, static_cast<uint32_t>(words), mg.data()));
+    LRN_HIP(hipEventCreateWithFlags(&a->start, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->ready, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->rest_done, hipEventDisableTiming));
+    *out = a.get();
+    per_dev[dev].push_back(std::move(a));
     return LRN_OK;
 }
 
 std::atomic<int> g_lookahead{0};
+// Update inside a block column after each inner panel: 0 = bc_update_kernel (64 x 64 vector
+// tiles), 1 = the matrix-core GEMM (128 x 128 tiles).
+std::atomic<int> g_block_update{0};
 
 // Factors block column [K, K+B) (rows K..n-1) inner panel by inner panel; each inner panel's
 // interchanges are applied inside the block column at once, and its L updates the rest of
@@ -631,9 +654,19 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
         }
         LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
         if (k + w < KB) {
+            // U12 in place, then A22 -= L21 U12 inside the block column
             LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
-            LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
-                A + (k + w) * lda + (k + w), lda, n - (k + w), KB - (k + w), w, s));
+            const long long rows = n - (k + w), cols = KB - (k + w);
+            if (g_block_update.load() == 1) {
+                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
+                    A + (k + w) * lda + (k + w), lda, rows, cols, w, s));
+                continue;
+            }
+            const dim3 grid(static_cast<unsigned>(std::max<long long>(1, (rows + kBcRows - 1) / kBcRows)),
+                static_cast<unsigned>((cols + kBcCols - 1) / kBcCols));
+            TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
+            hipLaunchKernelGGL((bc_update_kernel<T, W>), grid, dim3(256), 0, s, A, lda, n, k, w, k + w, KB);
+            LRN_HIP(hipGetLastError());
         }
     }
     return LRN_OK;
@@ -665,15 +698,45 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
     return LRN_OK;
 }
 
-// The blocked factorisation, right-looking with a look-ahead of one block column.
-// For block column K (already factored; its row maps are in ws.rowmap):
-//   GEMM stream : interchanges on columns [0, K) and on the far columns [KB+B2, n), their U12
-//                 rows, and the far part of A22 -= L21 * U12;
-//   caller stream: interchanges, U12 rows and the A22 update of the next block column
-//                 [KB, KB+B2) only (the critical path);
-//   panel stream : factorisation of that next block column, as soon as it is updated.
-// The panel and GEMM streams are restricted to disjoint sets of compute units so the panel's
-// cooperative workgroups always find room. Nothing waits on the host.
+// Single-stream order of the blocked factorisation: per block column, the interchanges on
+// the columns left and right of it, its U12 rows, the trailing update, then the next block
+// column's factorisation.
+template <typename T, int W>
+int getrf_blocked_plain(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd,
+    const CaluWs<float>* calu)
+{
+    const long long lda = n;
+    const long long NB = std::min<long long>(std::max<long long>(tuned_outer_nb(n), W), static_cast<long long>(W) * kMaxInner);
+    const int rm_stride = 1 + 4 * W;
+    unsigned base = 0;
+    unsigned cbase = 0;  // arrival counter of the tournament panels (cleared with the panel state)
+    const int set_stride = kMaxInner * (1 + 4 * kMaxW);
+    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
+    LRN_TRY(reset_panel_state(ws, s));
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s, calu, cbase)));
+    for (long long K = 0; K < n; K += NB) {
+        const long long B = std::min(NB, n - K);
+        const long long KB = K + B;
+        const int nmaps = static_cast<int>((B + W - 1) / W);
+        LRN_TRY(launch_laswp_multi<T>(A, lda, maps(K), nmaps, rm_stride, 2 * W, 0, K, KB, n, s));
+        if (KB >= n) break;
+        LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
+        LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
+            n - KB, B, upd, s));
+        LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s, calu, cbase)));
+    }
+    return LRN_OK;
+}
+
+// The blocked factorisation, right-looking, optionally with a look-ahead of one block column.
+// With the look-ahead, for block column K (already factored; its row maps are in ws.rowmap):
+//   gemm stream  : interchanges on columns [0, K) and on the far columns [KB+B2, n), their
+//                  U12 rows, and the far part of A22 -= L21 * U12;
+//   narrow stream: interchanges, U12 rows and the A22 update of the next block column
+//                  [KB, KB+B2) only (the critical path);
+//   panel stream : factorisation of that next block column, as soon as it is updated,
+// so the next block column's panels run while the far update of this one is still going.
+// Nothing waits on the host.
 template <typename T, int W>
 int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd = -1,
     const CaluWs<float>* calu = nullptr)
@@ -683,28 +746,39 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     const int rm_stride = 1 + 4 * W;
     unsigned base = 0;
     unsigned cbase = 0;  // arrival counter of the tournament panels (cleared with the panel state)
-    AsideStream* side = nullptr;
-    const bool lookahead = g_lookahead.load() != 0 && n > NB;
-    if (lookahead) LRN_TRY(aside_stream(n > 16384 ? 64 : 32, &side));
     const int set_stride = kMaxInner * (1 + 4 * kMaxW);
     auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
-    LRN_TRY(reset_panel_state(ws, s));
-    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s, calu, cbase)));
-    if (lookahead) LRN_HIP(hipEventRecord(side->done, s));
+    if (g_lookahead.load() == 0 || n <= NB) return getrf_blocked_plain<T, W>(A, n, piv, ws, s, upd, calu);
+    // compute units of the panel stream: a multiple of 8 (the same number on every XCD) and
+    // enough to hold every cooperative workgroup of the largest (first) panel at once
+    const int la_mode = g_lookahead.load();
+    int np = la_mode == 1 ? 32 : la_mode;
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    {
+        int need = 0;
+        unsigned base_q = 0;
+        LRN_TRY((launch_panel<T, W>(A, lda, 0, n, W, piv, ws, base_q, s, &need)));
+        np = std::max(np, (need + kXcds - 1) / kXcds * kXcds);
+    }
+    if (np > device_props(dev).cus / 2) return getrf_blocked_plain<T, W>(A, n, piv, ws, s, upd, calu);
+    AsideStream* side = nullptr;
+    LRN_TRY(aside_stream(np, &side));
+    std::lock_guard<std::mutex> side_lock(side->mu);
+    const int gemm_cus = device_props(dev).cus - side->np;
+    hipStream_t sn = side->narrow;
+    LRN_HIP(hipEventRecord(side->start, s));
+    LRN_HIP(hipStreamWaitEvent(sn, side->start, 0));
+    LRN_HIP(hipStreamWaitEvent(side->panel, side->start, 0));
+    LRN_HIP(hipStreamWaitEvent(side->gemm, side->start, 0));
+    LRN_TRY(reset_panel_state(ws, sn));
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, sn, calu, cbase)));
+    LRN_HIP(hipEventRecord(side->done, sn));
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
         const int nmaps = static_cast<int>((B + W - 1) / W);
         int* const rms = maps(K);
-        if (!lookahead) {
-            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, KB, n, s));
-            if (KB >= n) break;
-            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
-            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
-                n - KB, B, upd, s));
-            LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s, calu, cbase)));
-            continue;
-        }
         const long long B2 = std::min(NB, n - KB);  // next block column (0 at the end)
         const long long far0 = KB + B2;
         // far columns and the left columns: GEMM stream, after block column K is factored
@@ -713,24 +787,31 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         if (far0 < n) {
             LRN_TRY((u12_columns<T, W>(A, n, K, B, far0, n, side->gemm)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
-                n - far0, B, upd, side->gemm));
+                n - far0, B, upd, side->gemm, gemm_cus));
         }
         LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
-        // next block column: caller stream
-        LRN_HIP(hipStreamWaitEvent(s, side->done, 0));
+        // next block column: narrow stream, then its factorisation on the panel stream
+        LRN_HIP(hipStreamWaitEvent(sn, side->done, 0));
         if (B2 > 0) {
-            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, s));
-            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, s)));
+            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, sn));
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, sn)));
             LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
-                B, upd, s));
-            LRN_HIP(hipEventRecord(side->ready, s));
+                B, upd, sn));
+            LRN_HIP(hipEventRecord(side->ready, sn));
             LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
             LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel, calu, cbase)));
             LRN_HIP(hipEventRecord(side->done, side->panel));
         }
-        // the next step's caller-stream work needs this step's far update (its columns)
-        LRN_HIP(hipStreamWaitEvent(s, side->rest_done, 0));
+        // the next step's narrow work needs this step's far update (its columns)
+        LRN_HIP(hipStreamWaitEvent(sn, side->rest_done, 0));
     }
+    // hand back: the caller's stream continues once all three streams are done
+    LRN_HIP(hipEventRecord(side->ready, sn));
+    LRN_HIP(hipEventRecord(side->done, side->panel));
+    LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
+    LRN_HIP(hipStreamWaitEvent(s, side->ready, 0));
+    LRN_HIP(hipStreamWaitEvent(s, side->done, 0));
+    LRN_HIP(hipStreamWaitEvent(s, side->rest_done, 0));
     return LRN_OK;
 }
 
@@ -1687,9 +1768,17 @@ LRN_API int lrn_set_refine_solver(int chain)
     return LRN_OK;
 }
 
-LRN_API int lrn_set_lookahead(int enable)
+LRN_API int lrn_set_block_update(int kind)
 {
-    g_lookahead.store(enable ? 1 : 0);
+    if (kind < 0 || kind > 1) return -1;
+    g_block_update.store(kind);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_lookahead(int mode)
+{
+    if (mode < 0 || (mode > 1 && (mode < kXcds || mode % kXcds != 0))) return -1;
+    g_lookahead.store(mode);
     return LRN_OK;
 }
 

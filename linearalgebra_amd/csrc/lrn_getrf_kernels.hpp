@@ -395,6 +395,65 @@ __global__ __launch_bounds__(256) void trsm_llnu_kernel(T* __restrict__ A, long 
     }
 }
 
+// Update of the rest of a block column after one of its inner panels (rows k..k+w-1 and the
+// columns below them factored, U12 = L11^-1 A12 already solved in place by trsm_llnu_kernel):
+//   A22 -= L21 * U12   (U12: rows k..k+w-1, columns [c0, c1); A22: rows k+w..n-1).
+// The skinny GEMM of a block column (K = w, N <= the block width) as many small workgroups:
+// workgroup (x, y) owns the 64-row block x and the 64-column tile y of A22, stages its L21
+// rows and U12 tile in LDS and updates its block with 4 x 4 register tiles, each entry's
+// products accumulated from zero and subtracted once. It keeps its speed on the few compute
+// units of a look-ahead panel stream, where 128 x 128 matrix-core tiles leave most lanes idle.
+constexpr int kBcRows = 64, kBcCols = 64;
+template <typename T, int W>
+__global__ __launch_bounds__(256) void bc_update_kernel(T* __restrict__ A, long long lda, long long n, long long k,
+    int w, long long c0, long long c1)
+{
+    __shared__ T U[W][kBcCols + 1];
+    __shared__ T Lb[kBcRows][W + 1];
+    const int tid = threadIdx.x;
+    const long long cb = c0 + static_cast<long long>(blockIdx.y) * kBcCols;
+    const int ncols = static_cast<int>(min(static_cast<long long>(kBcCols), c1 - cb));
+    const long long rb = k + w + static_cast<long long>(blockIdx.x) * kBcRows;
+    const int nrows = static_cast<int>(max(0ll, min(static_cast<long long>(kBcRows), n - rb)));
+    for (int idx = tid; idx < kBcRows * W; idx += 256) {
+        const int r = idx / W, c = idx % W;
+        Lb[r][c] = (r < nrows && c < w) ? A[(rb + r) * lda + k + c] : T(0);
+    }
+    for (int idx = tid; idx < W * kBcCols; idx += 256) {
+        const int r = idx / kBcCols, c = idx % kBcCols;
+        U[r][c] = (r < w && c < ncols) ? A[(k + r) * lda + cb + c] : T(0);
+    }
+    __syncthreads();
+    // 16 x 16 threads, 4 x 4 entries each: rows 4*ty.., columns 4*tx..
+    const int tx = tid & 15, ty = tid >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+#pragma unroll 4
+    for (int p = 0; p < w; ++p) {
+        T l[4], u[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) l[i] = Lb[4 * ty + i][p];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[j] = U[p][4 * tx + j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fma(l[i], u[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = 4 * ty + i;
+        if (r >= nrows) continue;
+        T* crow = A + (rb + r) * lda + cb;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (4 * tx + j < ncols) crow[4 * tx + j] = crow[4 * tx + j] - acc[i][j];
+    }
+}
+
 // All row maps of a block column (nmaps maps, each the net moves of one inner panel, at most
 // W2 moves each, applied in order) on the columns [c0a, c1a) and [c0b, c1b) in one pass:
 // each workgroup takes a strip of kSwapBlkCols columns, gives every distinct row named by the

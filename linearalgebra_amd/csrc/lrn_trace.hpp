@@ -26,7 +26,8 @@ enum TraceKind {
     kTrTrsv = 8,         // forward / backward substitution of the solve, bytes
     kTrGemmBf16x6 = 9,   // gemm_sub_bf16_kernel<3>  (three planes), FP32-equivalent flops 2MNK
     kTrBatched = 10,     // batched_gesv_kernel (work not known on the host: 0)
-    kTrKinds = 11
+    kTrBcUpdate = 11,    // bc_update_kernel (A22 update inside a block column), flops
+    kTrKinds = 12
 };
 
 struct TraceRec {

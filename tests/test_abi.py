@@ -137,3 +137,22 @@ def test_tuning_file_is_applied(tmp_path):
         lib.lrn_set_panel_kernel(0)
         autotune.apply(lib)
     assert not autotune.apply(lib, str(tmp_path / "absent.json"))
+
+
+def test_tuning_setters_validate():
+    """Process-wide switches reject values outside their documented range (no GPU needed)."""
+    from linearalgebra_amd import _lib, autotune
+    lib = _lib.load()
+    try:
+        assert lib.lrn_set_lookahead(-1) != 0
+        assert lib.lrn_set_lookahead(5) != 0      # not a multiple of 8
+        assert lib.lrn_set_lookahead(12) != 0
+        assert lib.lrn_set_lookahead(16) == 0
+        assert lib.lrn_set_lookahead(1) == 0
+        assert lib.lrn_set_block_update(2) != 0
+        assert lib.lrn_set_block_update(-1) != 0
+        assert lib.lrn_set_block_update(1) == 0
+    finally:
+        lib.lrn_set_lookahead(0)
+        lib.lrn_set_block_update(0)
+        autotune.apply(lib)

@@ -208,3 +208,59 @@ def test_outer_block_widths_match_oracle(outer_nb, dev):
     assert info_g == info_o == 0 and ok
     assert np.array_equal(piv_g, piv_o), f"first mismatch at {O.first_pivot_mismatch(piv_g, piv_o)}"
     assert O.rel_diff(lu_g, lu_o) <= 1e-12
+
+
+@pytest.mark.parametrize("n,dtype", [(1100, "f64"), (2100, "f64"), (2100, "f32")])
+def test_lookahead_matches_plain_order(n, dtype, dev):
+    """The look-ahead only reorders launches over three streams (the next block column's
+    update and panels beside the rest of the trailing update); every entry sees the same
+    operations in the same order, so factors, pivots and the solution are bitwise those of
+    the single-stream order, which matches the oracle."""
+    from linearalgebra_amd import _lib
+    lib = _lib.load()
+    a = O.gaussian(n, seed=300 + n)
+    if dtype == "f32":
+        a = a.astype(np.float32)
+    b = O.gaussian(n, 1, seed=301 + n)[:, 0].astype(a.dtype)
+    out = {}
+    try:
+        for mode in (0, 1):
+            assert lib.lrn_set_lookahead(mode) == 0
+            out[mode] = (_gpu_getrf(a, dev), _gpu_solve(a, b, dev, mode=1))
+    finally:
+        from linearalgebra_amd import autotune
+        lib.lrn_set_lookahead(0)
+        autotune.apply(lib)
+    (lu0, piv0, info0, ok0), (_, _, x0) = out[0]
+    (lu1, piv1, info1, ok1), (_, _, x1) = out[1]
+    assert ok1 and info1 == info0 == 0
+    assert np.array_equal(piv1, piv0)
+    assert np.array_equal(lu1, lu0)
+    assert np.array_equal(x1, x0)
+    if dtype == "f64":
+        lu_o, piv_o, _ = O.getrf(a)
+        assert np.array_equal(piv1, piv_o), f"first mismatch at {O.first_pivot_mismatch(piv1, piv_o)}"
+        assert O.rel_diff(lu1, lu_o) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_block_update_kernels_match_oracle(kind, dev):
+    """Both block-column update kernels (vector tiles, matrix-core GEMM) give the oracle's
+    pivots and factors within rounding, with and without the look-ahead."""
+    from linearalgebra_amd import _lib, autotune
+    lib = _lib.load()
+    n = 1500
+    a = O.gaussian(n, seed=55)
+    lu_o, piv_o, _ = O.getrf(a)
+    try:
+        assert lib.lrn_set_block_update(kind) == 0
+        for mode in (0, 16):
+            assert lib.lrn_set_lookahead(mode) == 0
+            lu_g, piv_g, info_g, ok = _gpu_getrf(a, dev)
+            assert ok and info_g == 0
+            assert np.array_equal(piv_g, piv_o), f"first mismatch at {O.first_pivot_mismatch(piv_g, piv_o)}"
+            assert O.rel_diff(lu_g, lu_o) <= 1e-12
+    finally:
+        lib.lrn_set_block_update(0)
+        lib.lrn_set_lookahead(0)
+        autotune.apply(lib)
