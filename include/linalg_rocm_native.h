@@ -207,9 +207,14 @@ LRN_API int lrn_get_tuning(int* outer_nb, int* panel_w, int* panel_max_wg);
  * many; orders above 16384 use at least 64. Process-wide; the autotuner picks the value. */
 LRN_API int lrn_set_lookahead(int mode);
 /* Update of the rest of a block column after each of its inner panels: 0 = 64 x 64 tiles on
- * the vector units (the default), 1 = the matrix-core GEMM. Results differ only in rounding
- * of the update's sums. Process-wide; the autotuner picks the value. */
+ * the vector units (the default), 1 = the trailing-update GEMM (128 x 128 matrix-core tiles),
+ * 2 = 64 x 64 matrix-core tiles. Results differ only in rounding of the update's sums.
+ * Process-wide; the autotuner picks the value. */
 LRN_API int lrn_set_block_update(int kind);
+/* Refinement of lrn_gesv_mixed_*: 0 = LU-IR while every step at least halves the backward
+ * error, then GMRES-IR from the best iterate (the default); 1 = GMRES-IR right after the
+ * first LU-IR step. Process-wide. */
+LRN_API int lrn_set_refinement(int mode);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,

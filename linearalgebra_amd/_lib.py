@@ -46,6 +46,7 @@ SIGNATURES = {
     "lrn_set_tuning": [_int, _int, _int],
     "lrn_set_lookahead": [_int],
     "lrn_set_block_update": [_int],
+    "lrn_set_refinement": [_int],
     "lrn_set_mixed_update": [_int],
     "lrn_gesv_mixed_batched_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, ctypes.c_double, _int, _int, _vp, _sz,
                                    _vp, _vp],

@@ -100,7 +100,7 @@ def tune(n: int = 16384, batch_count: int = 10000, out: str = TUNING_PATH) -> di
     # the default 32-column register-resident panel
     for outer_nb in (128, 256):
         for lookahead in (0, 16, 32, 64):
-            for block_update in (0, 1):
+            for block_update in (0, 1, 2):
                 if (outer_nb, 32, 0, lookahead, block_update) not in configs:
                     configs.append((outer_nb, 32, 0, lookahead, block_update))
     for outer_nb, panel_w, panel_kernel, lookahead, block_update in configs:

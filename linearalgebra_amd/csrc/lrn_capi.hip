@@ -624,7 +624,8 @@ int aside_stream(int np, AsideStream** out)
 
 std::atomic<int> g_lookahead{0};
 // Update inside a block column after each inner panel: 0 = bc_update_kernel (64 x 64 vector
-// tiles), 1 = the matrix-core GEMM (128 x 128 tiles).
+// tiles), 1 = the trailing-update GEMM (128 x 128 matrix-core tiles), 2 = gemm_sub_kernel with
+// 64 x 64 matrix-core tiles.
 std::atomic<int> g_block_update{0};
 
 // Factors block column [K, K+B) (rows K..n-1) inner panel by inner panel; each inner panel's
@@ -657,9 +658,20 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
             // U12 in place, then A22 -= L21 U12 inside the block column
             LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
             const long long rows = n - (k + w), cols = KB - (k + w);
-            if (g_block_update.load() == 1) {
+            const int kind = g_block_update.load();
+            if (kind == 1) {
                 LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
                     A + (k + w) * lda + (k + w), lda, rows, cols, w, s));
+                continue;
+            }
+            if (kind == 2) {
+                // matrix cores, 64 x 64 tiles, the whole K = w in one LDS stage
+                const long long tm = (rows + 63) / 64, tn = (cols + 63) / 64;
+                TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
+                hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s,
+                    A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
+                    static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
+                LRN_HIP(hipGetLastError());
                 continue;
             }
             const dim3 grid(static_cast<unsigned>(std::max<long long>(1, (rows + kBcRows - 1) / kBcRows)),
@@ -690,10 +702,17 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
         return LRN_OK;
     }
     // one launch for the whole block column
-    const long long blocks = (c1 - c0 + kU12TN - 1) / kU12TN;
+    // narrow column ranges (the next block column): 16 columns per workgroup for parallelism
+    const bool narrow = c1 - c0 <= 256;
+    const int tn = narrow ? kU12TNNarrow : kU12TN;
+    const long long blocks = (c1 - c0 + tn - 1) / tn;
     TraceScope ts(kTrTrsm, (static_cast<double>(B) * B + 2.0 * B * static_cast<double>(c1 - c0)) * sizeof(T), s);
-    hipLaunchKernelGGL(u12_block_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, K,
-        static_cast<int>(B), c0, c1);
+    if (narrow)
+        hipLaunchKernelGGL((u12_block_kernel<T, kU12TNNarrow>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A,
+            lda, K, static_cast<int>(B), c0, c1);
+    else
+        hipLaunchKernelGGL((u12_block_kernel<T, kU12TN>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda,
+            K, static_cast<int>(B), c0, c1);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -849,6 +868,9 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
 // preconditioner): 1 = 256-row chained kernel, 0 = the 64-row kernel (the default: the
 // 256-row kernel streams its factor rows through too few compute units to keep up).
 std::atomic<int> g_chain_solve{0};
+// Refinement of the mixed solver: 0 = LU-IR while each step gains at least 2x, then GMRES-IR;
+// 1 = GMRES-IR right after the first LU-IR step.
+std::atomic<int> g_refinement{0};
 
 // Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
 // without touching B (LU mode after a singular factorisation).
@@ -1346,13 +1368,17 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
             }
             // stagnating (less than a 2x gain over the best so far) or diverging: GMRES-IR
             if (it >= 1 && !(berr <= 0.5 * prev_best)) break;
+            if (g_refinement.load() == 1) break;  // GMRES-IR from the first iterate
         }
         if (!done) {
             // restart GMRES-IR from the best LU-IR iterate
             LRN_TRY(S.copy(x, L.xbest));
             LRN_TRY(S.residual(&berr));
             for (int outer = 0; outer < max_iter && !done; ++outer) {
-                LRN_TRY(S.gmres(m, 1e-10, &st.gmres_iterations));
+                // reduce the preconditioned residual by what the target asks for, with a
+                // factor 10 of margin (within [1e-12, 1e-2])
+                const double rtol = std::min(1e-2, std::max(1e-12, 0.1 * tol / berr));
+                LRN_TRY(S.gmres(m, rtol, &st.gmres_iterations));
                 LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
                 ++st.gmres_outer;
                 LRN_TRY(S.residual(&berr));
@@ -1768,9 +1794,16 @@ LRN_API int lrn_set_refine_solver(int chain)
     return LRN_OK;
 }
 
+LRN_API int lrn_set_refinement(int mode)
+{
+    if (mode < 0 || mode > 1) return -1;
+    g_refinement.store(mode);
+    return LRN_OK;
+}
+
 LRN_API int lrn_set_block_update(int kind)
 {
-    if (kind < 0 || kind > 1) return -1;
+    if (kind < 0 || kind > 2) return -1;
     g_block_update.store(kind);
     return LRN_OK;
 }

@@ -402,7 +402,9 @@ __global__ __launch_bounds__(256) void trsm_llnu_kernel(T* __restrict__ A, long 
 // workgroup (x, y) owns the 64-row block x and the 64-column tile y of A22, stages its L21
 // rows and U12 tile in LDS and updates its block with 4 x 4 register tiles, each entry's
 // products accumulated from zero and subtracted once. It keeps its speed on the few compute
-// units of a look-ahead panel stream, where 128 x 128 matrix-core tiles leave most lanes idle.
+// units of a look-ahead panel stream, where 128 x 128 matrix-core tiles leave most lanes
+// idle. (The same update on the matrix cores with 64 x 64 tiles is gemm_sub_kernel<T, 64,
+// 64, 32>; the host picks one.)
 constexpr int kBcRows = 64, kBcCols = 64;
 template <typename T, int W>
 __global__ __launch_bounds__(256) void bc_update_kernel(T* __restrict__ A, long long lda, long long n, long long k,
@@ -568,13 +570,16 @@ __global__ __launch_bounds__(256) void laswp_block_kernel(T* __restrict__ A, lon
 // the 32 x 32 diagonal block. Every entry receives its updates in increasing pivot order with
 // each product rounded before the subtraction, which is exactly the row-wise forward
 // substitution of src/matrix_operation/cpu/native/lu_decomposition_native.cpp:364-380, so the
-// result does not depend on the grouping.
-constexpr int kU12TN = 64;
-template <typename T>
+// result does not depend on the grouping, nor on TN (columns per workgroup: 64 for wide
+// column ranges; 16 for the next block column alone, which is on the look-ahead's critical
+// path and would otherwise give only a few workgroups).
+constexpr int kU12TN = 64, kU12TNNarrow = 16;
+template <typename T, int TN>
 __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long long lda, long long K, int B,
     long long c0, long long c1)
 {
-    constexpr int TN = kU12TN;
+    static_assert(TN % 8 == 0 && TN <= 64, "8 threads per row, TN / 8 columns each");
+    constexpr int CPT = TN / 8;
     __shared__ T U[256][TN + 1];
     __shared__ T Ld[32][33];
     __shared__ T Lc[2][32][33];  // 32 x 32 pieces of the rows' L, double-buffered
@@ -582,16 +587,16 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
     const long long cb = c0 + static_cast<long long>(blockIdx.x) * TN;
     const int ncols = static_cast<int>(min(static_cast<long long>(TN), c1 - cb));
     const int nin = (B + 31) / 32;
-    const int r = tid >> 3, cg = (tid & 7) * 8;
+    const int r = tid >> 3, cg = (tid & 7) * CPT;
     const int lr = tid >> 3, lc = (tid & 7) * 4;  // staging: 4 consecutive L entries per thread
     for (int p = 0; p < nin; ++p) {
         const long long r0 = K + 32 * p;
         const int rp = min(32, B - 32 * p);
-        T x[8];
+        T x[CPT];
         if (r < rp) {
             const T* arow = A + (r0 + r) * lda + cb + cg;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) x[e] = (cg + e < ncols) ? arow[e] : T(0);
+            for (int e = 0; e < CPT; ++e) x[e] = (cg + e < ncols) ? arow[e] : T(0);
         }
         // contributions of the earlier groups, 32 pivots at a time
         T st[4];
@@ -612,13 +617,13 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
                 for (int qq = 0; qq < 32; ++qq) {
                     const T l = Lc[buf][r][qq];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) x[e] = x[e] - l * U[32 * qc + qq][cg + e];
+                    for (int e = 0; e < CPT; ++e) x[e] = x[e] - l * U[32 * qc + qq][cg + e];
                 }
             }
         }
         if (r < rp) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) U[32 * p + r][cg + e] = x[e];
+            for (int e = 0; e < CPT; ++e) U[32 * p + r][cg + e] = x[e];
         }
         for (int idx = tid; idx < 32 * 32; idx += 256) {
             const int i = idx >> 5, q = idx & 31;

@@ -19,14 +19,14 @@ enum TraceKind {
     kTrGemmF64Edge = 1,  // gemm_sub_f64_kernel<true>   (ragged strips), flops
     kTrGemmBf16x3 = 2,   // gemm_sub_bf16_kernel<2>  (two planes), FP32-equivalent flops 2MNK
     kTrGemmBf16 = 3,     // gemm_sub_bf16_kernel<1>, flops
-    kTrGemmF32 = 4,      // gemm_sub_kernel<float>, flops
+    kTrGemmF32 = 4,      // gemm_sub_f32_kernel, flops
     kTrPanel = 5,        // panel factorisation, flops of the panel (rank-1 updates)
     kTrLaswp = 6,        // row interchanges, bytes (every pivot taken as an interchange)
     kTrTrsm = 7,         // unit-lower TRSM of U12 rows, bytes
     kTrTrsv = 8,         // forward / backward substitution of the solve, bytes
     kTrGemmBf16x6 = 9,   // gemm_sub_bf16_kernel<3>  (three planes), FP32-equivalent flops 2MNK
     kTrBatched = 10,     // batched_gesv_kernel (work not known on the host: 0)
-    kTrBcUpdate = 11,    // bc_update_kernel (A22 update inside a block column), flops
+    kTrBcUpdate = 11,    // update inside a block column (bc_update_kernel / 64x64 gemm_sub_kernel), flops
     kTrKinds = 12
 };
 

@@ -149,10 +149,14 @@ def test_tuning_setters_validate():
         assert lib.lrn_set_lookahead(12) != 0
         assert lib.lrn_set_lookahead(16) == 0
         assert lib.lrn_set_lookahead(1) == 0
-        assert lib.lrn_set_block_update(2) != 0
+        assert lib.lrn_set_block_update(3) != 0
+        assert lib.lrn_set_block_update(2) == 0
         assert lib.lrn_set_block_update(-1) != 0
         assert lib.lrn_set_block_update(1) == 0
+        assert lib.lrn_set_refinement(2) != 0
+        assert lib.lrn_set_refinement(1) == 0
     finally:
+        lib.lrn_set_refinement(0)
         lib.lrn_set_lookahead(0)
         lib.lrn_set_block_update(0)
         autotune.apply(lib)

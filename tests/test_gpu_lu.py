@@ -243,10 +243,11 @@ def test_lookahead_matches_plain_order(n, dtype, dev):
         assert O.rel_diff(lu1, lu_o) <= 1e-12
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_block_update_kernels_match_oracle(kind, dev):
-    """Both block-column update kernels (vector tiles, matrix-core GEMM) give the oracle's
-    pivots and factors within rounding, with and without the look-ahead."""
+    """Every block-column update kernel (vector tiles, matrix-core GEMM with 128 x 128 or
+    64 x 64 tiles) gives the oracle's pivots and factors within rounding, with and without
+    the look-ahead."""
     from linearalgebra_amd import _lib, autotune
     lib = _lib.load()
     n = 1500

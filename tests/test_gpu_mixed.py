@@ -149,3 +149,28 @@ def test_gesv_mixed_batched_uneven(dev):
         assert st["path"] in (0, 1, 2)
         assert fe <= max(1e-12, kappa[i] * k * np.finfo(np.float64).eps)
     assert la is not None
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+def test_gesv_mixed_gmres_first(kappa, dev):
+    """Refinement mode 1 (GMRES-IR right after the first LU-IR step) reaches the FP64
+    backward error on the GMRES path."""
+    import torch
+    import linearalgebra_amd as la
+    from linearalgebra_amd import _lib
+    lib = _lib.load()
+    n = 1000
+    a = O.conditioned(n, kappa, seed=31)
+    b = O.gaussian(n, 1, seed=32).ravel()
+    try:
+        assert lib.lrn_set_refinement(1) == 0
+        x, st = la.gesv_mixed(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev), tol=1e-14, max_iter=30,
+                              restart=40)
+    finally:
+        lib.lrn_set_refinement(0)
+    berr = O.backward_error(a, x.cpu().numpy(), b)
+    print(f"gmres-first kappa={kappa:.0e} path={st['path_name']} ir={st['ir_iterations']} "
+          f"gmres={st['gmres_iterations']} berr={berr:.2e}", flush=True)
+    assert berr <= 2e-14
+    assert st["ir_iterations"] == 1 or st["path"] == 0
+    assert st["path"] in (0, 1)
