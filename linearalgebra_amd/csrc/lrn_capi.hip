@@ -370,22 +370,6 @@ int launch_calu(float* A, long long lda, long long k, long long m, int w, long l
 }
 
 template <typename T>
-int launch_laswp(T* A, long long lda, const int* rowmap, int W2, long long c0a, long long c1a, long long c0b,
-    long long c1b, hipStream_t s)
-{
-    const long long total = (c1a - c0a) + (c1b - c0b);
-    if (total <= 0) return LRN_OK;
-    const long long blocks = (total + kSwapCols - 1) / kSwapCols;
-    const size_t lds = static_cast<size_t>(W2) * kSwapCols * sizeof(T);
-    // bytes if every pivot of the panel is an interchange (W2 / 2 swaps, 2 rows read + written each)
-    TraceScope ts(kTrLaswp, 2.0 * W2 * static_cast<double>(total) * sizeof(T), s);
-    hipLaunchKernelGGL(laswp_rowmap_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s, A, lda, rowmap,
-        W2, c0a, c1a, c0b, c1b);
-    LRN_HIP(hipGetLastError());
-    return LRN_OK;
-}
-
-template <typename T>
 int launch_laswp_multi(T* A, long long lda, const int* rowmaps, int nmaps, int map_stride, int W2, long long c0a,
     long long c1a, long long c0b, long long c1b, hipStream_t s)
 {
@@ -419,6 +403,21 @@ int launch_trsm(T* A, long long lda, long long k, int w, long long c0, long long
     TraceScope ts(kTrTrsm, (static_cast<double>(w) * w + 2.0 * w * static_cast<double>(c1 - c0)) * sizeof(T), s);
     hipLaunchKernelGGL((trsm_llnu_kernel<T, W>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda, k, w, c0,
         c1);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T, int W>
+int launch_laswp_trsm(T* A, long long lda, const int* rowmap, int W2, long long K, long long k, int w, long long KB,
+    hipStream_t s)
+{
+    const long long na = (k - K + kSwapCols - 1) / kSwapCols;
+    const long long nb = (KB - (k + w) + kSwapCols - 1) / kSwapCols;
+    if (na + nb <= 0) return LRN_OK;
+    const double cols = static_cast<double>((k - K) + (KB - (k + w)));
+    TraceScope ts(kTrLaswp, (2.0 * W2 * cols + static_cast<double>(w) * w + 2.0 * w * (KB - k - w)) * sizeof(T), s);
+    hipLaunchKernelGGL((laswp_trsm_kernel<T, W>), dim3(static_cast<unsigned>(na + nb)), dim3(256), 0, s, A, lda, rowmap,
+        W2, K, k, w, KB, static_cast<int>(na));
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -653,10 +652,10 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
             (void)cbase;
             LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
         }
-        LRN_TRY(launch_laswp<T>(A, lda, pw.rowmap, 2 * W, K, k, k + w, KB, s));
+        // the panel's row moves on the block column's other columns and its U12 rows
+        LRN_TRY((launch_laswp_trsm<T, W>(A, lda, pw.rowmap, 2 * W, K, k, w, KB, s)));
         if (k + w < KB) {
-            // U12 in place, then A22 -= L21 U12 inside the block column
-            LRN_TRY((launch_trsm<T, W>(A, lda, k, w, k + w, KB, s)));
+            // A22 -= L21 U12 inside the block column
             const long long rows = n - (k + w), cols = KB - (k + w);
             const int kind = g_block_update.load();
             if (kind == 1) {

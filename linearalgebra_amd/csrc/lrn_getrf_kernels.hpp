@@ -3,7 +3,8 @@
 //   getf2_panel      cooperative panel factorisation of an m x w column block: pivot search by
 //                    wave-shuffle argmax, row swaps and rank-1 updates on an LDS-resident tile,
 //                    one bounded inter-workgroup barrier per column
-//   laswp_rowmap     applies one panel's row interchanges to the columns outside the panel
+//   laswp_trsm       one panel's row interchanges on the rest of its block column, and its
+//                    U12 rows
 //   trsm_llnu        U12 = L11^-1 A12 for a unit-lower w x w diagonal block (w <= 64)
 //   gemm_sub         C -= A * B on the matrix cores (FP64 / FP32 MFMA 16x16x4)
 //
@@ -264,40 +265,62 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel_kernel(T* __restric
     if (b == 0 && tid < kWave) panel_rowmap(pivs, k, w, 2 * W, ws.rowmap);
 }
 
-// Applies the net row moves of one panel (rowmap) to the columns [c0a, c1a) and [c0b, c1b).
-// Each workgroup moves a strip of kSwapCols columns: gather every source row into LDS, then
-// scatter to the destination rows, so a row that is both read and written is never raced.
+// Row moves are applied in strips of kSwapCols columns per workgroup: gather every source
+// row into LDS, then scatter to the destination rows, so a row that is both read and written
+// is never raced.
 constexpr int kSwapCols = 64;
-template <typename T>
-__global__ __launch_bounds__(256) void laswp_rowmap_kernel(T* __restrict__ A, long long lda,
-    const int* __restrict__ rowmap, int W2, long long c0a, long long c1a, long long c0b, long long c1b)
+// After an inner panel (rows k.., columns k..k+w-1) of block column [K, KB): its row moves on
+// the block column's other columns and U12 = L11^-1 A12 of rows k..k+w-1, in one launch
+// (a row-move kernel followed by trsm_llnu_kernel would be two latency-bound launches on
+// little data).
+// Workgroups [0, na) take 64-column strips of [K, k) (moves only), workgroups [na, ..) strips
+// of [k+w, KB): moves, a barrier (the workgroup's own stores are then visible to it), then
+// one thread per column solves the strip's U12 rows in LDS in increasing pivot order with
+// every product rounded before its subtraction -- bitwise the result of trsm_llnu_kernel.
+template <typename T, int W>
+__global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long long lda,
+    const int* __restrict__ rowmap, int W2, long long K, long long k, int w, long long KB, int na)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* buf = reinterpret_cast<T*>(lds_raw);  // [cnt][kSwapCols]
+    __shared__ T buf[2 * W][kSwapCols];
+    __shared__ T L11[W][W + 1];
+    __shared__ T X[W][kSwapCols + 1];
+    const int tid = threadIdx.x;
+    const bool right = static_cast<int>(blockIdx.x) >= na;
+    const long long cb = right ? k + w + static_cast<long long>(blockIdx.x - na) * kSwapCols
+                               : K + static_cast<long long>(blockIdx.x) * kSwapCols;
+    const long long ce = right ? KB : k;
+    const int ncols = static_cast<int>(min(static_cast<long long>(kSwapCols), ce - cb));
     const int cnt = rowmap[0];
     const int* dst = rowmap + 1;
     const int* src = rowmap + 1 + W2;
-    const long long la = c1a - c0a;
-    const long long total = la + (c1b - c0b);
-    const long long s = static_cast<long long>(blockIdx.x) * kSwapCols;
-    for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
-        const int q = idx / kSwapCols;
-        const int cc = idx - q * kSwapCols;
-        const long long t = s + cc;
-        if (t < total) {
-            const long long col = (t < la) ? c0a + t : c0b + (t - la);
-            buf[idx] = A[static_cast<long long>(src[q]) * lda + col];
-        }
+    for (int idx = tid; idx < cnt * kSwapCols; idx += 256) {
+        const int q = idx / kSwapCols, c = idx - q * kSwapCols;
+        if (c < ncols) buf[q][c] = A[static_cast<long long>(src[q]) * lda + cb + c];
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt * kSwapCols; idx += blockDim.x) {
-        const int q = idx / kSwapCols;
-        const int cc = idx - q * kSwapCols;
-        const long long t = s + cc;
-        if (t < total) {
-            const long long col = (t < la) ? c0a + t : c0b + (t - la);
-            A[static_cast<long long>(dst[q]) * lda + col] = buf[idx];
+    for (int idx = tid; idx < cnt * kSwapCols; idx += 256) {
+        const int q = idx / kSwapCols, c = idx - q * kSwapCols;
+        if (c < ncols) A[static_cast<long long>(dst[q]) * lda + cb + c] = buf[q][c];
+    }
+    if (!right) return;  // uniform over the workgroup
+    __syncthreads();
+    for (int idx = tid; idx < W * W; idx += 256) {
+        const int r = idx / W, c = idx % W;
+        L11[r][c] = (r < w && c < w) ? A[(k + r) * lda + k + c] : T(0);
+    }
+    for (int idx = tid; idx < W * kSwapCols; idx += 256) {
+        const int r = idx / kSwapCols, c = idx % kSwapCols;
+        X[r][c] = (r < w && c < ncols) ? A[(k + r) * lda + cb + c] : T(0);
+    }
+    __syncthreads();
+    if (tid < ncols) {
+        const int c = tid;
+        for (int i = 1; i < w; ++i) {
+            T s = X[i][c];
+            for (int p = 0; p < i; ++p) s = s - L11[i][p] * X[p][c];
+            X[i][c] = s;
         }
+        for (int i = 1; i < w; ++i) A[(k + i) * lda + cb + c] = X[i][c];
     }
 }
 
