@@ -19,6 +19,12 @@ summed over launches, divided by the summed launch durations). `phases` gives ev
 kind from one extra traced step. `cpu_baseline` is the CPU restatement of the reference's
 native algorithm timed on this host on a smaller order and scaled by n^3.
 
+Configuration grids (skipped with --no-configs): `plain_configs` times the plain FP64 LU +
+solve at n = 4096, 8192, 16384 with nrhs 1 and 64 on A iid N(0,1) and on A + n*I, each with
+its backward error; `mixed_configs` runs the mixed solve at the headline order for kappa =
+1e2 and 1e10 (the headline is kappa = 1e6), with path, iteration counts, backward error and
+forward difference from the plain FP64 solution (--mixed-extra-n 32768 adds that order).
+
 Usage: python bench.py [--steps K] [--warmup W] [--n N] [--kappa K] [--cpu-n N] [--no-cpu]
 """
 from __future__ import annotations
@@ -68,6 +74,13 @@ def parse():
     p.add_argument("--batch-replays", type=int, default=20)
     p.add_argument("--batch-mixed-count", type=int, default=64,
                    help="systems of the batched mixed-precision workload (0 = skip)")
+    p.add_argument("--no-configs", action="store_true", help="skip the configuration grids")
+    p.add_argument("--config-sizes", default="4096,8192,16384",
+                   help="orders of the plain FP64 grid (nrhs 1 and 64, A iid N(0,1) and A + n I)")
+    p.add_argument("--mixed-kappas", default="1e2,1e10",
+                   help="further condition numbers of the mixed grid at the headline order")
+    p.add_argument("--mixed-extra-n", type=int, default=0,
+                   help="also run the mixed grid (all three condition numbers) at this order, e.g. 32768")
     p.add_argument("--mixed-update", type=int, default=3,
                    help="trailing update of the mixed factorisation: 0/1/2 = 1/2/3 bf16 planes, 3 = FP32 MFMA")
     return p.parse_args()
@@ -162,6 +175,104 @@ def backward_error_gpu(torch, a, x, b):
     r = b - a @ x
     den = torch.linalg.matrix_norm(a, ord=float("inf")) * x.abs().max() + b.abs().max()
     return float(r.abs().max() / den)
+
+
+def backward_error_cols(torch, a, x, b):
+    """Largest normwise backward error over the right-hand sides (columns of x, b)."""
+    x2 = x.reshape(x.shape[0], -1)
+    b2 = b.reshape(b.shape[0], -1)
+    r = b2 - a @ x2
+    anorm = torch.linalg.matrix_norm(a, ord=float("inf"))
+    den = anorm * x2.abs().amax(dim=0) + b2.abs().amax(dim=0)
+    return float((r.abs().amax(dim=0) / den).max())
+
+
+# ----------------------------------------------------------------------------- configs
+
+def plain_configs_bench(torch, sizes, nrhs_list, seed, dev, reps=3):
+    """Plain FP64 LU + solve over the grid of the FP64 GPU workload: n in `sizes`, nrhs in
+    `nrhs_list`, A iid N(0,1) and A + n*I, B iid N(0,1). Per case: median of `reps` timed
+    solves after one warmup (HIP events around the solver call on the launch stream; the
+    restore copies of A and B are outside) and the backward error of the result."""
+    import linearalgebra_amd as la
+    lu_mode = la.linear_solver_type.LU_LINEAR_SOLVER
+    out = []
+    for n in sizes:
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed + n)
+        a0 = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        piv = torch.empty(n, dtype=torch.int64, device=dev)
+        aw = torch.empty_like(a0)
+        for variant in ("gaussian", "diag_dominant"):
+            a = a0 if variant == "gaussian" else a0 + n * torch.eye(n, dtype=torch.float64, device=dev)
+            for nrhs in nrhs_list:
+                b = torch.randn(n, nrhs, dtype=torch.float64, device=dev, generator=g)
+                if nrhs == 1:
+                    b = b.reshape(n).contiguous()
+                x = torch.empty_like(b)
+                ws = torch.empty(la.linear_solver.workspace_size(n, lu_mode, torch.float64, nrhs), dtype=torch.uint8,
+                                 device=dev)
+                times = []
+                for r in range(reps + 1):
+                    aw.copy_(a)
+                    x.copy_(b)
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    la.linear_solver.apply(aw, piv, n, x, lu_mode, ws)
+                    e1.record()
+                    torch.cuda.synchronize()
+                    if r > 0:
+                        times.append(e0.elapsed_time(e1))
+                ms = sorted(times)[len(times) // 2]
+                berr = backward_error_cols(torch, a, x, b)
+                out.append({"n": n, "matrix": variant, "nrhs": nrhs, "ms": round(ms, 3),
+                            "solves_per_s": round(1e3 / ms, 3), "lu_tflops": round(2.0 * n ** 3 / 3.0 / (ms * 1e-3) / 1e12, 2),
+                            "backward_error": berr})
+                log(f"plain n={n} {variant} nrhs={nrhs}: {ms:.2f} ms, berr {berr:.2e}")
+                del ws, x, b
+            if variant == "diag_dominant":
+                del a
+        del a0, aw, piv
+    return out
+
+
+def mixed_configs_bench(torch, sizes, kappas, seed, dev, restart):
+    """Mixed LU-IR over the grid of the mixed workload (n in `sizes`, kappa in `kappas`, A = U
+    diag(s) V^T as the headline input): one warmup and one timed solve each, the refinement
+    path and iteration counts, the backward error, and the forward difference from the plain
+    FP64 GPU LU solution of the same system."""
+    import linearalgebra_amd as la
+    from linearalgebra_amd import mixed as lam
+    lu_mode = la.linear_solver_type.LU_LINEAR_SOLVER
+    out = []
+    for n in sizes:
+        ws = torch.empty(lam.workspace_size(n, restart, True), dtype=torch.uint8, device=dev)
+        for kappa in kappas:
+            a, b = conditioned_gpu(torch, n, kappa, seed, dev)
+            x = torch.empty(n, dtype=torch.float64, device=dev)
+            la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=restart, workspace=ws)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, st = la.gesv_mixed(a, b, x, tol=1e-14, max_iter=30, restart=restart, workspace=ws)
+            torch.cuda.synchronize()
+            ms = 1e3 * (time.perf_counter() - t0)
+            berr = backward_error_gpu(torch, a, x, b)
+            aw = a.clone()
+            xp = b.clone()
+            piv = torch.empty(n, dtype=torch.int64, device=dev)
+            la.linear_solver.apply(aw, piv, n, xp, lu_mode)
+            fwd = float((x - xp).abs().max() / xp.abs().max())
+            del aw, piv
+            out.append({"n": n, "kappa": kappa, "ms": round(ms, 2), "solves_per_s": round(1e3 / ms, 3),
+                        "path": ["lu-ir", "gmres-ir", "fp64-fallback", "not-converged"][st["path"]],
+                        "ir_iterations": st["ir_iterations"], "gmres_iterations": st["gmres_iterations"],
+                        "factor_ms": round(st["factor_ms"], 2), "refine_ms": round(st["refine_ms"], 2),
+                        "backward_error": berr, "forward_difference_vs_fp64_lu": fwd})
+            log(f"mixed n={n} kappa={kappa:g}: {ms:.1f} ms path={out[-1]['path']} berr {berr:.2e} fwd {fwd:.1e}")
+            del a, b, x, xp
+        del ws
+    return out
 
 
 # ----------------------------------------------------------------------------- batched
@@ -417,6 +528,17 @@ def main():
                  "lu_tflops": round(2.0 * n ** 3 / 3.0 / (t_plain / steps) / 1e12, 2)}
         del aw, wsp
 
+    plain_configs = None
+    mixed_configs = None
+    if not args.no_configs:
+        plain_configs = plain_configs_bench(torch, [int(v) for v in args.config_sizes.split(",") if v],
+                                            (1, 64), args.seed, dev)
+        kappas = [float(v) for v in args.mixed_kappas.split(",") if v]
+        mixed_configs = mixed_configs_bench(torch, [n], kappas, args.seed, dev, args.restart) if kappas else []
+        if args.mixed_extra_n > 0:
+            mixed_configs += mixed_configs_bench(torch, [args.mixed_extra_n], [1e2, args.kappa] + [k for k in kappas if k > 1e6],
+                                                 args.seed, dev, args.restart)
+
     batched = None
     if not args.no_batched:
         batched = batched_bench(torch, args.batch_count, args.batch_replays, args.seed, dev)
@@ -466,6 +588,8 @@ def main():
         "roofline_fp64_lu": roof_fp64,
         "phases_mixed": phases_mixed,
         "phases_plain_fp64": phases_plain,
+        "plain_configs": plain_configs,
+        "mixed_configs": mixed_configs,
         "batched": batched,
         "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,

@@ -89,6 +89,30 @@ __device__ __forceinline__ void wait_counter(const unsigned* ctr, unsigned targe
     }
 }
 
+// Bounded wait for a flag published with publish_flag, without an acquire fence: the poll
+// and the data loads that follow it are agent-coherent (sc1) accesses, and the data loads
+// are issued only after the poll has returned the flag, so no cache invalidation is needed
+// (an agent-scope acquire invalidates the XCD's L2 on every poll).
+__device__ __forceinline__ void wait_flag_coherent(const unsigned* flag, unsigned target, unsigned limit, int* err)
+{
+    unsigned spins = 0;
+    while (__hip_atomic_load(const_cast<unsigned*>(flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        ++spins;
+        if ((spins & 255u) == 0u && error_raised(err)) break;
+        if (spins > limit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Completes this thread's outstanding memory operations (on this GPU family stores are
+// counted in vmcnt). Called by every thread after its agent-coherent (sc1) data stores and
+// followed by a workgroup barrier, so one thread can then publish a flag that covers them all
+// without a release fence (an agent-scope release writes back the XCD's L2).
+__device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ void signal_counter(unsigned* ctr)
 {
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
             const int r = idx / TB, c = idx % TB;
             Ls[r][c] = (r < rows && c < cols) ? static_cast<T>(LU[(r0 + r) * lda + c0 + c]) : T(0);
         }
-        if (tid == 0) wait_counter(&flags[j], 1u, spin_limit, err);
+        if (tid == 0) wait_flag_coherent(&flags[j], 1u, spin_limit, err);
         __syncthreads();
         for (int idx = tid; idx < TB * nc; idx += 256) {
             const int r = idx / nc, c = idx % nc;
@@ -212,9 +212,9 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
         const int r = idx / nc, c = idx % nc;
         st_agent(&Out[(r0 + r) * ld_out + c], Xs[r][c]);
     }
-    __threadfence();
+    wait_own_stores();
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[b], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(&flags[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Single right-hand side (the refinement's and the plain solve's case), latency-optimised.
