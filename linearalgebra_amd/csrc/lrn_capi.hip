@@ -1373,6 +1373,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
             // restart GMRES-IR from the best LU-IR iterate
             LRN_TRY(S.copy(x, L.xbest));
             LRN_TRY(S.residual(&berr));
+            double gbest = berr;
             for (int outer = 0; outer < max_iter && !done; ++outer) {
                 // reduce the preconditioned residual by what the target asks for, with a
                 // factor 10 of margin (within [1e-12, 1e-2])
@@ -1387,7 +1388,21 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
                 if (berr <= tol) {
                     st.path = 1;
                     done = true;
+                    break;
                 }
+                const double prev = gbest;
+                if (berr < gbest) {
+                    LRN_TRY(S.copy(L.xbest, x));
+                    gbest = berr;
+                }
+                // a restart cycle that does not halve the best backward error: GMRES-IR has
+                // stalled (the low-precision factors are too far from A); stop here
+                if (outer >= 1 && !(berr <= 0.5 * prev)) break;
+            }
+            if (!done) {
+                // leave the best iterate in x
+                LRN_TRY(S.copy(x, L.xbest));
+                berr = gbest;
             }
         }
     }
