@@ -144,8 +144,9 @@ LRN_API int lrn_set_mixed_update(int mode);
  * path (the default). Either way the factors only precondition the FP64 refinement. */
 LRN_API int lrn_set_mixed_panel(int tournament);
 /* Triangular solves of the mixed solver's refinement (FP32 factors applied in FP64,
- * process-wide, no reference counterpart): 1 = 256-row workgroups chained through tagged
- * hand-offs, 0 = the 64-row kernel of the plain single-RHS solve (the default). */
+ * process-wide, no reference counterpart): 0 = the 64-row kernel of the plain single-RHS
+ * solve (the default); 1 = 256-row, 2 = 128-row workgroups chained through tagged hand-offs,
+ * each solving its rows in 64-row sub-blocks. */
 LRN_API int lrn_set_refine_solver(int chain);
 /* Kernel of the 32-column cooperative panel (process-wide, no reference counterpart; same
  * pivots and factors either way): 0 = panel rows held in registers, two workgroup barriers

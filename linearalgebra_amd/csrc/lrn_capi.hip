@@ -864,8 +864,8 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
 }
 
 // Single-RHS substitution with FP32 factors in FP64 arithmetic (the refinement's
-// preconditioner): 1 = 256-row chained kernel, 0 = the 64-row kernel (the default: the
-// 256-row kernel streams its factor rows through too few compute units to keep up).
+// preconditioner): 0 = the 64-row kernel (the default), 1 = 256-row chained kernel (streams
+// its factor rows through too few compute units to keep up), 2 = 128-row chained kernel.
 std::atomic<int> g_chain_solve{0};
 // Refinement of the mixed solver: 0 = LU-IR while each step gains at least 2x, then GMRES-IR;
 // 1 = GMRES-IR right after the first LU-IR step.
@@ -906,23 +906,33 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
             const double tri_bytes = (0.5 * static_cast<double>(n) * n) * sizeof(TF) + 4.0 * n * sizeof(T);
             TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
             if constexpr (std::is_same<TF, float>::value && std::is_same<T, double>::value) {
-                if (g_chain_solve.load() != 0 && ldb == 1) {
-                    const unsigned nb256 = static_cast<unsigned>((n + kChainTB - 1) / kChainTB);
+                const int chain = g_chain_solve.load();
+                if (chain != 0 && ldb == 1) {
                     const bool vec4 = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(LU) % 16 == 0);
-                    if (vec4) {
-                        hipLaunchKernelGGL((trsv_chain_kernel<T, true, true>), dim3(nb256), dim3(256), 0, s, LU, n, n,
-                            L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                    auto go = [&](auto ns) -> int {
+                        constexpr int NS = decltype(ns)::value;
+                        constexpr int TB = kChainSB * NS;
+                        const unsigned nbk = static_cast<unsigned>((n + TB - 1) / TB);
+                        if (vec4) {
+                            hipLaunchKernelGGL((trsv_chain_kernel<T, true, true, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
+                                n, L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                            LRN_HIP(hipGetLastError());
+                            hipLaunchKernelGGL((trsv_chain_kernel<T, false, true, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
+                                n, L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                        } else {
+                            hipLaunchKernelGGL((trsv_chain_kernel<T, true, false, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
+                                n, L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                            LRN_HIP(hipGetLastError());
+                            hipLaunchKernelGGL((trsv_chain_kernel<T, false, false, NS>), dim3(nbk), dim3(TB), 0, s, LU,
+                                n, n, L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                        }
                         LRN_HIP(hipGetLastError());
-                        hipLaunchKernelGGL((trsv_chain_kernel<T, false, true>), dim3(nb256), dim3(256), 0, s, LU, n, n,
-                            L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
-                    } else {
-                        hipLaunchKernelGGL((trsv_chain_kernel<T, true, false>), dim3(nb256), dim3(256), 0, s, LU, n, n,
-                            L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
-                        LRN_HIP(hipGetLastError());
-                        hipLaunchKernelGGL((trsv_chain_kernel<T, false, false>), dim3(nb256), dim3(256), 0, s, LU, n, n,
-                            L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
-                    }
-                    LRN_HIP(hipGetLastError());
+                        return LRN_OK;
+                    };
+                    if (chain == 2)
+                        LRN_TRY(go(std::integral_constant<int, 2>{}));
+                    else
+                        LRN_TRY(go(std::integral_constant<int, 4>{}));
                     continue;
                 }
             }
@@ -1804,7 +1814,8 @@ LRN_API int lrn_set_panel_kernel(int kind)
 
 LRN_API int lrn_set_refine_solver(int chain)
 {
-    g_chain_solve.store(chain ? 1 : 0);
+    if (chain < 0 || chain > 2) return -1;
+    g_chain_solve.store(chain);
     return LRN_OK;
 }
 

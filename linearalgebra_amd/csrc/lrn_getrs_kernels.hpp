@@ -364,24 +364,29 @@ __global__ __launch_bounds__(256) void trsv1_kernel(const TF* __restrict__ LU, l
 //    this one solves the rest.
 // Lower pass: rows in increasing order, unit diagonal. Upper pass: decreasing order,
 // q = x * (1/d) with one residual correction.
-constexpr int kChainTB = 256;
 constexpr int kChainSB = 64;
 
-__device__ __forceinline__ int chain_piece(int s, int sp, bool lower)
+// Slot of the off-diagonal piece (s, s') inside an NS-sub-block workgroup block; lower:
+// s' < s; upper: s' > s, mirrored onto the same slots.
+template <int NS> __device__ __forceinline__ int chain_piece(int s, int sp, bool lower)
 {
-    // lower: (s, s') with s' < s; upper: (s, s') with s' > s, mirrored onto the same slots
-    const int a = lower ? s : 3 - s, c = lower ? sp : 3 - sp;
+    const int a = lower ? s : NS - 1 - s, c = lower ? sp : NS - 1 - sp;
     return a * (a - 1) / 2 + c;
 }
 
-template <typename T, bool kLower, bool kVec4>
-__global__ __launch_bounds__(256) void trsv_chain_kernel(const float* __restrict__ LU, long long lda, long long n,
-    const int* __restrict__ perm, const T* __restrict__ Bin, T* __restrict__ Out, Chunk* __restrict__ xch,
-    unsigned tag, unsigned spin_limit, int* __restrict__ err, const int* __restrict__ skip_if_nonzero)
+// NS sub-blocks of 64 rows per workgroup (NS * 64 threads, one row each): 4 (256 rows) or
+// 2 (128 rows), which halves the per-workgroup stream of factor rows for the same halving
+// of hand-offs.
+template <typename T, bool kLower, bool kVec4, int NS>
+__global__ __launch_bounds__(kChainSB * NS) void trsv_chain_kernel(const float* __restrict__ LU, long long lda,
+    long long n, const int* __restrict__ perm, const T* __restrict__ Bin, T* __restrict__ Out,
+    Chunk* __restrict__ xch, unsigned tag, unsigned spin_limit, int* __restrict__ err,
+    const int* __restrict__ skip_if_nonzero)
 {
     if (skip_if_nonzero != nullptr && *skip_if_nonzero != 0) return;
-    constexpr int TB = kChainTB, SB = kChainSB, NS = TB / SB;
-    __shared__ float Lp[6][SB][SB + 1];
+    constexpr int SB = kChainSB, TB = NS * SB, NP = NS * (NS - 1) / 2;
+    static_assert(NS >= 2 && NS <= 4, "2 to 4 sub-blocks");
+    __shared__ float Lp[NP][SB][SB + 1];
     __shared__ T Xs[2][SB];
     __shared__ T Xd[NS][SB];
     __shared__ T Pp[TB][17];
@@ -399,14 +404,14 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const float* __restrict
     const long long row = r0 + tid;
     const float* lrow = LU + (row_ok ? row : r0) * lda;
 
-    for (int idx = tid; idx < 6 * SB * SB; idx += 256) {
+    for (int idx = tid; idx < NP * SB * SB; idx += TB) {
         const int piece = idx / (SB * SB);
         const int rem = idx - piece * SB * SB;
         const int rr = rem / SB, cc = rem - rr * SB;
         // piece -> (a, c) with c < a in the mirrored numbering
         const int a = piece < 1 ? 1 : (piece < 3 ? 2 : 3);
         const int c = piece - a * (a - 1) / 2;
-        const int s = kLower ? a : 3 - a, sp = kLower ? c : 3 - c;
+        const int s = kLower ? a : NS - 1 - a, sp = kLower ? c : NS - 1 - c;
         const long long gr = r0 + s * SB + rr, gc = r0 + sp * SB + cc;
         Lp[piece][rr][cc] = (gr < n && gc < n) ? LU[gr * lda + gc] : 0.f;
     }
@@ -539,7 +544,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const float* __restrict
         __syncthreads();
         const bool later = kLower ? (wid > s) : (wid < s);
         if (later && wid < nsub) {
-            const int piece = chain_piece(wid, s, kLower);
+            const int piece = chain_piece<NS>(wid, s, kLower);
 #pragma unroll
             for (int e0 = 0; e0 < SB; e0 += 16) {
 #pragma unroll

@@ -77,10 +77,11 @@ def getrf_tournament(m: torch.Tensor, pivot: torch.Tensor | None = None, stream=
     return pivot, int(info.cpu().item())
 
 
-def set_refine_solver(chain: bool) -> None:
-    """Triangular solves of the refinement (process-wide): True = 256-row chained kernel,
-    False = the 64-row kernel of the plain single-RHS solve (the default)."""
-    _check(_lib.load().lrn_set_refine_solver(int(bool(chain))), "set_refine_solver")
+def set_refine_solver(chain: int) -> None:
+    """Triangular solves of the refinement (process-wide): 0 / False = the 64-row kernel of the
+    plain single-RHS solve (the default), 1 / True = 256-row chained kernel, 2 = 128-row
+    chained kernel."""
+    _check(_lib.load().lrn_set_refine_solver(int(chain)), "set_refine_solver")
 
 
 def getrs_f32f64(lu: torch.Tensor, pivot: torch.Tensor, b: torch.Tensor, stream=None) -> torch.Tensor:
