@@ -81,6 +81,24 @@ LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double
 LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* x, int type,
     void* workspace, size_t workspace_size, void* stream);
 
+/* ---- triangular_solver<T, ROCM_NATIVE> ------------------------------------------------
+ * Replaces triangular_solver<T, B>::apply / workspace_size
+ * (include/matrix_operation/triangular_solver.h:20-38): op(A) X = B, A n x n triangular
+ * (upper or lower, op = transpose or not, unit diagonal implicit and not read), B and X
+ * n x nrhs, all row-major with row strides lda, ldb, ldx; only the selected triangle of A
+ * is read; A and B are preserved; A, B, X distinct device buffers. Argument errors as the
+ * reference's checks (src/common/triangular_solver_checks.h:11-35), in argument order
+ * (-4 n, -5 nrhs, -6 a, -7 lda, -8 b, -9 ldb, -10 x, -11 ldx, -12 workspace). Singular A is
+ * not diagnosed. ASYNCHRONOUS on stream. */
+LRN_API int lrn_triangular_solver_workspace_size_f64(int64_t n, int64_t nrhs, size_t* bytes);
+LRN_API int lrn_triangular_solver_workspace_size_f32(int64_t n, int64_t nrhs, size_t* bytes);
+LRN_API int lrn_triangular_solver_f64(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const double* a,
+    int64_t lda, const double* b, int64_t ldb, double* x, int64_t ldx, void* workspace, size_t workspace_size,
+    void* stream);
+LRN_API int lrn_triangular_solver_f32(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const float* a,
+    int64_t lda, const float* b, int64_t ldb, float* x, int64_t ldx, void* workspace, size_t workspace_size,
+    void* stream);
+
 /* ---- multi-RHS extension (no reference counterpart: the reference solves one RHS;
  * docs/backend_roadmap.md:285 lists multi-RHS as future work) ---------------------------
  * Same semantics as linear_solver with B a row-major lda x nrhs block (row stride ldb >=

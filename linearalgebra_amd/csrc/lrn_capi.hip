@@ -969,6 +969,73 @@ int getrs_run(const T* LU, long long n, const long long* piv, T* B, long long nr
 }
 
 constexpr long long kMaxOrder = 1ll << 17;
+// ------------------------------------------------------------- triangular solver
+// op(A) X = B with A triangular (reference: include/matrix_operation/triangular_solver.h:11-38,
+// algorithm src/matrix_operation/cpu/native/triangular_solver_native.cpp:64-153): rows are
+// solved in 64-row blocks chained between workgroups (trsv_blocked_kernel with the
+// direction, unit-diagonal and transpose flags), 64 right-hand sides per launch.
+size_t trsm_ws_bytes(long long n)
+{
+    Arena ar;
+    ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
+    ar.next<int>(64);
+    return ar.off;
+}
+
+template <typename T, bool kFwd, bool kUnit, bool kTrans>
+int launch_trsm_chain(const T* a, long long lda, long long n, const T* b, long long ldb, T* x, long long ldx, int nc,
+    unsigned* flags, int* err, hipStream_t s)
+{
+    hipLaunchKernelGGL((trsv_blocked_kernel<T, T, kFwd, kUnit, kTrans>), dim3(static_cast<unsigned>(solve_blocks(n))),
+        dim3(256), 0, s, a, lda, n, nullptr, b, ldb, x, ldx, nc, flags, kSpinLimit, err, nullptr);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T>
+int triangular_solver_impl(int upper, int transpose, int unit, long long n, long long nrhs, const T* a, long long lda,
+    const T* b, long long ldb, T* x, long long ldx, void* ws, size_t ws_size, hipStream_t s)
+{
+    // the reference's checks (src/common/triangular_solver_checks.h:11-35), in argument order
+    if (n <= 0) return -4;
+    if (nrhs <= 0) return -5;
+    if (a == nullptr) return -6;
+    if (lda < n) return -7;
+    if (b == nullptr || b == a) return -8;
+    if (ldb < nrhs) return -9;
+    if (x == nullptr || x == a || x == b) return -10;
+    if (ldx < nrhs) return -11;
+    if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (ws == nullptr || ws_size < trsm_ws_bytes(n)) return -12;
+    Arena ar(ws, ws_size);
+    unsigned* flags = ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
+    int* err = ar.next<int>(64);
+    // rows in increasing order when op(A) is lower triangular
+    const bool fwd = (upper != 0) == (transpose != 0);
+    const bool u = unit != 0, t = transpose != 0;
+    TraceScope ts(kTrTrsv, (0.5 * static_cast<double>(n) * n + 2.0 * static_cast<double>(n) * nrhs) * sizeof(T), s);
+    for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
+        const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
+        LRN_TRY(launch_fill(flags, 0u, solve_blocks(n), s));
+        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
+        const T* bb = b + c0;
+        T* xx = x + c0;
+        int rc;
+        if (fwd)
+            rc = u ? (t ? launch_trsm_chain<T, true, true, true>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s)
+                        : launch_trsm_chain<T, true, true, false>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s))
+                   : (t ? launch_trsm_chain<T, true, false, true>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s)
+                        : launch_trsm_chain<T, true, false, false>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s));
+        else
+            rc = u ? (t ? launch_trsm_chain<T, false, true, true>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s)
+                        : launch_trsm_chain<T, false, true, false>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s))
+                   : (t ? launch_trsm_chain<T, false, false, true>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s)
+                        : launch_trsm_chain<T, false, false, false>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s));
+        LRN_TRY(rc);
+    }
+    return LRN_OK;
+}
+
 
 template <typename T>
 int lu_decomposition_impl(T* m, int64_t lda, int64_t* pivot, void* workspace, size_t ws_size, int* info, void* stream)
@@ -1582,6 +1649,31 @@ LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* 
     size_t workspace_size, void* stream)
 {
     return solver_impl<float>(m, pivot, lda, x, 1, 1, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_triangular_solver_workspace_size_f64(int64_t n, int64_t nrhs, size_t* bytes)
+{
+    if (n <= 0) return -1;
+    if (nrhs <= 0) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::trsm_ws_bytes(n);
+    return LRN_OK;
+}
+LRN_API int lrn_triangular_solver_workspace_size_f32(int64_t n, int64_t nrhs, size_t* bytes)
+{
+    return lrn_triangular_solver_workspace_size_f64(n, nrhs, bytes);
+}
+LRN_API int lrn_triangular_solver_f64(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const double* a,
+    int64_t lda, const double* b, int64_t ldb, double* x, int64_t ldx, void* workspace, size_t workspace_size, void* stream)
+{
+    return lrn::triangular_solver_impl<double>(upper, transpose, unit, n, nrhs, a, lda, b, ldb, x, ldx, workspace,
+        workspace_size, static_cast<hipStream_t>(stream));
+}
+LRN_API int lrn_triangular_solver_f32(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const float* a,
+    int64_t lda, const float* b, int64_t ldb, float* x, int64_t ldx, void* workspace, size_t workspace_size, void* stream)
+{
+    return lrn::triangular_solver_impl<float>(upper, transpose, unit, n, nrhs, a, lda, b, ldb, x, ldx, workspace,
+        workspace_size, static_cast<hipStream_t>(stream));
 }
 
 LRN_API int lrn_linear_solver_multi_workspace_size_f64(int64_t lda, int64_t nrhs, int type, size_t* bytes)

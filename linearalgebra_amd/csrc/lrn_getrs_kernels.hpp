@@ -87,6 +87,10 @@ template <> __device__ __forceinline__ float read_lane_t<float>(float v, int lan
 //
 // Forward:  Y[b] = L[b][b]^-1 ( B[perm][b] - sum_{j<b} L[b][j] Y[j] )   (unit lower)
 // Backward: X[b] = U[b][b]^-1 ( Y[b]       - sum_{j>b} U[b][j] X[j] )   (B receives X)
+// In general kLower picks the direction (increasing rows), kUnit an implicit unit diagonal
+// (otherwise a true division), kTrans reads the operator transposed (entry (i, j) is
+// A[j][i]), and perm == nullptr the identity: the triangular solver of the library uses it
+// with the reference's four flag combinations.
 // `nc` right-hand-side columns starting at column c0 of B / Y (row strides ldb / ldy).
 // The factors have type TF and are converted to the working type T as they are staged
 // (TF = float, T = double applies low-precision factors in FP64 arithmetic).
@@ -96,7 +100,7 @@ template <> __device__ __forceinline__ float read_lane_t<float>(float v, int lan
 // side, the earlier off-diagonal blocks -- is loaded or applied before the wait, and the
 // diagonal solve keeps the block row in registers and broadcasts each solved value with a
 // lane read instead of an LDS round trip.
-template <typename TF, typename T, bool kLower>
+template <typename TF, typename T, bool kLower, bool kUnit = kLower, bool kTrans = false>
 __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict__ LU, long long lda, long long n,
     const int* __restrict__ perm, const T* __restrict__ Bin, long long ldb_in, T* __restrict__ Out,
     long long ld_out, int nc, unsigned* __restrict__ flags, unsigned spin_limit, int* __restrict__ err,
@@ -116,10 +120,14 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
     const long long r0 = b * TB;
     const int rows = static_cast<int>(min(static_cast<long long>(TB), n - r0));
 
-    // independent of every other block: diagonal block and right-hand side
+    // entry (i, j) of the triangular operator: A[i][j], or A[j][i] when transposed
+    auto coef = [&](long long i, long long j) { return static_cast<T>(kTrans ? LU[j * lda + i] : LU[i * lda + j]); };
+    // independent of every other block: diagonal block (its own triangle only; a unit
+    // diagonal is not read) and right-hand side
     for (int idx = tid; idx < TB * TB; idx += 256) {
         const int r = idx / TB, c = idx % TB;
-        Ds[r][c] = (r < rows && c < rows) ? static_cast<T>(LU[(r0 + r) * lda + r0 + c]) : T(0);
+        const bool in = (r < rows && c < rows) && (kLower ? (c < r || (!kUnit && c == r)) : (c > r || (!kUnit && c == r)));
+        Ds[r][c] = in ? coef(r0 + r, r0 + c) : T(0);
     }
     T rhs[kRhsRegs];
 #pragma unroll
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
         const int r = nc > 0 ? idx / nc : 0, c = nc > 0 ? idx % nc : 0;
         rhs[k] = T(0);
         if (idx < TB * nc && r < rows) {
-            const long long src = kLower ? static_cast<long long>(perm[r0 + r]) : (r0 + r);
+            const long long src = (kLower && perm != nullptr) ? static_cast<long long>(perm[r0 + r]) : (r0 + r);
             rhs[k] = Bin[src * ldb_in + c];
         }
     }
@@ -148,7 +156,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
         // stage the L/U block while the producer may still be running
         for (int idx = tid; idx < TB * TB; idx += 256) {
             const int r = idx / TB, c = idx % TB;
-            Ls[r][c] = (r < rows && c < cols) ? static_cast<T>(LU[(r0 + r) * lda + c0 + c]) : T(0);
+            Ls[r][c] = (r < rows && c < cols) ? coef(r0 + r, c0 + c) : T(0);
         }
         if (tid == 0) wait_flag_coherent(&flags[j], 1u, spin_limit, err);
         __syncthreads();
@@ -190,6 +198,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
 #pragma unroll
                 for (int p = 0; p < TB; ++p) {
                     if (p < rows) {
+                        if (!kUnit && lane == p) x = x / drow[p];
                         const T xp = read_lane_t<T>(x, p);
                         if (lane > p) x = x - drow[p] * xp;
                     }
@@ -198,7 +207,7 @@ __global__ __launch_bounds__(256) void trsv_blocked_kernel(const TF* __restrict_
 #pragma unroll
                 for (int p = TB - 1; p >= 0; --p) {
                     if (p < rows) {
-                        if (lane == p) x = x / drow[p];
+                        if (!kUnit && lane == p) x = x / drow[p];
                         const T xp = read_lane_t<T>(x, p);
                         if (lane < p) x = x - drow[p] * xp;
                     }

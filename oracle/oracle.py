@@ -7,6 +7,10 @@ Reference correspondence:
   getrs   -> lu_solve, one column at a time       (src/matrix_operation/cpu/native/linear_solver_native.cpp:90-113)
   gesv    -> linear_solver<T, NATIVE>::apply, LU  (linear_solver_native.cpp:160-171)
   workspace_size -> lu_workspace_size<T>          (lu_decomposition_native.cpp:293-326)
+  trsm    -> triangular_solver<T, NATIVE>::apply  (src/matrix_operation/cpu/native/triangular_solver_native.cpp:64-153),
+             restated in numpy (scalar-path order); pinned by the reference's known-answer
+             generator (Testing/Cxx/triangular_solver_test_cases.h:12-68, restated as
+             trsm_known_answer_cases) and scipy's solve_triangular
 """
 from __future__ import annotations
 
@@ -255,3 +259,84 @@ __all__ = [
     "pivots_to_perm", "unpack_lu", "reconstruction_error", "backward_error", "forward_error",
     "rel_diff", "first_pivot_mismatch",
 ]
+
+
+# ------------------------------------------------------------------- triangular solver
+
+_TRSM_BLOCK = 32  # triangular_solver_native.cpp:17
+
+
+def trsm(upper: bool, transpose: bool, unit: bool, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """op(A) X = B the way triangular_solver_native.cpp:64-153 does it (scalar path): rows in
+    32-row blocks from the end op(A)'s triangle starts at; inside a block, row i takes
+    x_i -= coefficient(i, k) * x_k for the block's earlier rows k (in increasing k), then
+    x_i /= coefficient(i, i) unless unit; after the block, the rows still to do take
+    update = op(A)[rows, block] @ X[block] (i-k-j order, multiplication_native.cpp:1703-1720)
+    and x -= update. a is n x lda (only the triangle is read), b is n x nrhs."""
+    n = a.shape[0]
+    dt = a.dtype.type
+    x = np.array(b[:, :], dtype=a.dtype, copy=True)
+    nrhs = x.shape[1]
+
+    def coef(i, j):
+        return a[j, i] if transpose else a[i, j]
+
+    eff_upper = upper != transpose
+    done = 0
+    while done < n:
+        width = min(_TRSM_BLOCK, n - done)
+        begin = n - done - width if eff_upper else done
+        end = begin + width
+        for step in range(width):
+            i = end - step - 1 if eff_upper else begin + step
+            ks = range(i + 1, end) if eff_upper else range(begin, i)
+            for k in ks:
+                x[i] = x[i] - coef(i, k) * x[k]
+            if not unit:
+                x[i] = x[i] / coef(i, i)
+        first = 0 if eff_upper else end
+        rows = begin if eff_upper else n - end
+        if rows > 0:
+            update = np.zeros((rows, nrhs), dtype=a.dtype)
+            for r in range(rows):
+                for k in range(width):
+                    update[r] = update[r] + coef(first + r, begin + k) * x[begin + k]
+            x[first:first + rows] = x[first:first + rows] - update
+        done += width
+    return x.astype(dt, copy=False)
+
+
+def trsm_known_answer_cases(dtype=np.float64):
+    """The reference's known-answer generator for the triangular solver
+    (Testing/Cxx/triangular_solver_test_cases.h:12-68), restated: every (n, nrhs) of the list,
+    padded strides lda = n + 2, ldb = nrhs + 1, ldx = nrhs + 3, all eight flag combinations;
+    A is NaN outside the selected triangle (and on the diagonal when unit), the expected X is
+    sin(i * nrhs + j + 1), and B = op(A) X summed in increasing k in the working type.
+    Yields (upper, transpose, unit, n, nrhs, lda, ldb, ldx, a, b, expected) with a, b,
+    expected the padded row-major arrays."""
+    T = np.dtype(dtype).type
+    for n, nrhs in ((1, 1), (5, 3), (17, 7), (35, 1), (35, 9), (65, 2)):
+        lda, ldb, ldx = n + 2, nrhs + 1, nrhs + 3
+        for upper in (False, True):
+            for transpose in (False, True):
+                for unit in (False, True):
+                    a = np.full((n, lda), np.nan, dtype=dtype)
+                    expected = np.full((n, ldx), T(-123), dtype=dtype)
+                    for i in range(n):
+                        for j in range(n):
+                            if (j >= i) if upper else (j <= i):
+                                a[i, j] = (np.nan if unit else T(2.0 + 0.02 * i)) if i == j else T(0.01 * np.sin(float(i * n + j + 1)))
+                        for j in range(nrhs):
+                            expected[i, j] = T(np.sin(float(i * nrhs + j + 1)))
+                    b = np.full((n, ldb), T(-123), dtype=dtype)
+                    for i in range(n):
+                        for j in range(nrhs):
+                            acc = T(0)
+                            for k in range(n):
+                                row, col = (k, i) if transpose else (i, k)
+                                if (col < row) if upper else (col > row):
+                                    continue
+                                c = T(1) if (unit and row == col) else a[row, col]
+                                acc = T(acc + T(c * expected[k, j]))
+                            b[i, j] = acc
+                    yield upper, transpose, unit, n, nrhs, lda, ldb, ldx, a, b, expected
