@@ -23,7 +23,9 @@ Configuration grids (skipped with --no-configs): `plain_configs` times the plain
 solve at n = 4096, 8192, 16384 with nrhs 1 and 64 on A iid N(0,1) and on A + n*I, each with
 its backward error; `mixed_configs` runs the mixed solve at the headline order for kappa =
 1e2 and 1e10 (the headline is kappa = 1e6), with path, iteration counts, backward error and
-forward difference from the plain FP64 solution (--mixed-extra-n 32768 adds that order).
+forward difference from the plain FP64 solution (--mixed-extra-n 32768 adds that order);
+`triangular_solver` times op(A) X = B at the headline order for nrhs 1 and 64 and four flag
+combinations (HBM rate of the algorithmic bytes, residual).
 
 Usage: python bench.py [--steps K] [--warmup W] [--n N] [--kappa K] [--cpu-n N] [--no-cpu]
 """
@@ -273,6 +275,49 @@ def mixed_configs_bench(torch, sizes, kappas, seed, dev, restart):
             del a, b, x, xp
         del ws
     return out
+
+
+def triangular_bench(torch, n, seed, dev, reps=3):
+    """triangular_solver<double, ROCM_NATIVE> on an n x n triangle (entries N(0,1)/sqrt(n) off
+    the diagonal, 2 + U[0,1) on it) with nrhs 1 and 64, four flag combinations: median of
+    `reps` HIP-event-timed solves after a warmup, HBM rate of the algorithmic bytes
+    ((n^2/2 + 2 n nrhs) * 8) against the 8 TB/s peak, and the relative residual."""
+    import linearalgebra_amd as la
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed + 77)
+    a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g) / n ** 0.5
+    a += torch.diag(2.0 + torch.rand(n, dtype=torch.float64, device=dev, generator=g))
+    out = []
+    for nrhs in (1, 64):
+        b = torch.randn(n, nrhs, dtype=torch.float64, device=dev, generator=g)
+        x = torch.empty_like(b)
+        ws = torch.empty(la.triangular_solver.workspace_size(n, nrhs), dtype=torch.uint8, device=dev)
+        for upper, transpose, unit in ((False, False, True), (True, False, False), (False, True, False), (True, True, True)):
+            times = []
+            for r in range(reps + 1):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                la.triangular_solver.apply(upper, transpose, unit, n, nrhs, a, n, b, nrhs, x, nrhs, ws)
+                e1.record()
+                torch.cuda.synchronize()
+                if r > 0:
+                    times.append(e0.elapsed_time(e1))
+            ms = sorted(times)[len(times) // 2]
+            t = torch.triu(a) if upper else torch.tril(a)
+            if unit:
+                t.fill_diagonal_(1.0)
+            op = t.T if transpose else t
+            res = float((op @ x - b).abs().max() / ((op.abs().sum(1).max()) * x.abs().max() + b.abs().max()))
+            gbs = (0.5 * n * n + 2.0 * n * nrhs) * 8 / (ms * 1e-3) / 1e9
+            out.append({"nrhs": nrhs, "upper": upper, "transpose": transpose, "unit": unit, "ms": round(ms, 3),
+                        "gbs": round(gbs, 1), "hbm_frac": round(gbs / 8000.0, 4), "backward_error": res})
+            log(f"triangular n={n} nrhs={nrhs} upper={upper} transpose={transpose} unit={unit}: {ms:.2f} ms, "
+                f"{gbs:.0f} GB/s, berr {res:.1e}")
+            del t, op
+        del b, x, ws
+    del a
+    return {"n": n, "cases": out}
 
 
 # ----------------------------------------------------------------------------- batched
@@ -530,11 +575,13 @@ def main():
 
     plain_configs = None
     mixed_configs = None
+    triangular = None
     if not args.no_configs:
         plain_configs = plain_configs_bench(torch, [int(v) for v in args.config_sizes.split(",") if v],
                                             (1, 64), args.seed, dev)
         kappas = [float(v) for v in args.mixed_kappas.split(",") if v]
         mixed_configs = mixed_configs_bench(torch, [n], kappas, args.seed, dev, args.restart) if kappas else []
+        triangular = triangular_bench(torch, n, args.seed, dev)
         if args.mixed_extra_n > 0:
             mixed_configs += mixed_configs_bench(torch, [args.mixed_extra_n], [1e2, args.kappa] + [k for k in kappas if k > 1e6],
                                                  args.seed, dev, args.restart)
@@ -590,6 +637,7 @@ def main():
         "phases_plain_fp64": phases_plain,
         "plain_configs": plain_configs,
         "mixed_configs": mixed_configs,
+        "triangular_solver": triangular,
         "batched": batched,
         "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,

@@ -1,6 +1,6 @@
 /*
  * C++ face of the MI355X-native LU solve path: the reference's operator templates
- * (lu_decomposition<T, B>, linear_solver<T, B>) specialised for a new backend tag
+ * (lu_decomposition<T, B>, linear_solver<T, B>, triangular_solver<T, B>) specialised for a new backend tag
  * linalg_backend::ROCM_NATIVE. Same names, argument meaning, ownership and error
  * behaviour as the reference; the bodies (csrc/rocm_native_backend.cpp) call the C ABI of
  * linalg_rocm_native.h.
@@ -10,6 +10,7 @@
  *   include/common/types.h:18-29                       linalg_int (64-bit signed)
  *   include/matrix_operation/lu_decomposition.h:15-54  lu_decomposition<T, B>
  *   include/matrix_operation/linear_solver.h:15-65     linear_solver_type, linear_solver<T, B>
+ *   include/matrix_operation/triangular_solver.h:20-38 triangular_solver<T, B>
  *
  * Inside the reference tree, define LRN_WITH_REFERENCE_HEADERS and put the reference's
  * include root on the include path: the reference's own declarations are used and only the
@@ -31,6 +32,7 @@
 #include "include/common/types.h"
 #include "include/matrix_operation/linear_solver.h"
 #include "include/matrix_operation/lu_decomposition.h"
+#include "include/matrix_operation/triangular_solver.h"
 #else
 
 #if !defined(linalg_int)
@@ -71,6 +73,15 @@ template <typename T, linalg_backend Backend> struct linear_solver
         size_t workspace_size, void* stream = nullptr);
     static size_t workspace_size(linalg_int lda, linear_solver_type type);
 };
+
+// include/matrix_operation/triangular_solver.h:20-38
+template <typename T, linalg_backend Backend> struct triangular_solver
+{
+    static void apply(bool upper, bool transpose, bool unit_diagonal, linalg_int n, linalg_int nrhs, const T* a,
+        linalg_int lda, const T* b, linalg_int ldb, T* x, linalg_int ldx, void* workspace, size_t workspace_size,
+        void* stream = nullptr);
+    static size_t workspace_size(linalg_int n, linalg_int nrhs);
+};
 }  // namespace linalg
 #endif
 
@@ -94,6 +105,18 @@ template <> size_t linear_solver<double, linalg_backend::ROCM_NATIVE>::workspace
     linear_solver_type type);
 template <> size_t linear_solver<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
     linear_solver_type type);
+
+// triangular_solver: op(A) X = B, device pointers, asynchronous on `stream`.
+template <> void triangular_solver<double, linalg_backend::ROCM_NATIVE>::apply(bool upper, bool transpose,
+    bool unit_diagonal, linalg_int n, linalg_int nrhs, const double* a, linalg_int lda, const double* b,
+    linalg_int ldb, double* x, linalg_int ldx, void* workspace, size_t workspace_size, void* stream);
+template <> void triangular_solver<float, linalg_backend::ROCM_NATIVE>::apply(bool upper, bool transpose,
+    bool unit_diagonal, linalg_int n, linalg_int nrhs, const float* a, linalg_int lda, const float* b,
+    linalg_int ldb, float* x, linalg_int ldx, void* workspace, size_t workspace_size, void* stream);
+template <> size_t triangular_solver<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int n,
+    linalg_int nrhs);
+template <> size_t triangular_solver<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int n,
+    linalg_int nrhs);
 
 // Extensions of this backend (no reference counterpart).
 namespace rocm_native

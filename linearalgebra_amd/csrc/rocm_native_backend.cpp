@@ -42,6 +42,8 @@ template <> struct ops<double> {
     static constexpr auto sv = lrn_linear_solver_f64;
     static constexpr auto svm_ws = lrn_linear_solver_multi_workspace_size_f64;
     static constexpr auto svm = lrn_linear_solver_multi_f64;
+    static constexpr auto tr_ws = lrn_triangular_solver_workspace_size_f64;
+    static constexpr auto tr = lrn_triangular_solver_f64;
 };
 template <> struct ops<float> {
     static constexpr auto lu_ws = lrn_lu_decomposition_workspace_size_f32;
@@ -50,6 +52,8 @@ template <> struct ops<float> {
     static constexpr auto sv = lrn_linear_solver_f32;
     static constexpr auto svm_ws = lrn_linear_solver_multi_workspace_size_f32;
     static constexpr auto svm = lrn_linear_solver_multi_f32;
+    static constexpr auto tr_ws = lrn_triangular_solver_workspace_size_f32;
+    static constexpr auto tr = lrn_triangular_solver_f32;
 };
 
 template <typename T> bool lu_apply(T* m, linalg_int lda, linalg_int* pivot, void* ws, size_t ws_size, int* info, void* stream)
@@ -130,6 +134,43 @@ template <> size_t linear_solver<float, linalg_backend::ROCM_NATIVE>::workspace_
     linear_solver_type type)
 {
     return solve_ws<float>(lda, type);
+}
+
+template <typename T>
+void tr_apply(bool upper, bool transpose, bool unit, linalg_int n, linalg_int nrhs, const T* a, linalg_int lda,
+    const T* b, linalg_int ldb, T* x, linalg_int ldx, void* ws, size_t ws_size, void* stream)
+{
+    check(ops<T>::tr(upper ? 1 : 0, transpose ? 1 : 0, unit ? 1 : 0, n, nrhs, a, lda, b, ldb, x, ldx, ws, ws_size, stream),
+        "triangular_solver<ROCM_NATIVE>::apply");
+}
+template <typename T> size_t tr_ws(linalg_int n, linalg_int nrhs)
+{
+    size_t bytes = 0;
+    check(ops<T>::tr_ws(n, nrhs, &bytes), "triangular_solver<ROCM_NATIVE>::workspace_size");
+    return bytes;
+}
+
+template <> void triangular_solver<double, linalg_backend::ROCM_NATIVE>::apply(bool upper, bool transpose,
+    bool unit_diagonal, linalg_int n, linalg_int nrhs, const double* a, linalg_int lda, const double* b,
+    linalg_int ldb, double* x, linalg_int ldx, void* workspace, size_t workspace_size, void* stream)
+{
+    tr_apply<double>(upper, transpose, unit_diagonal, n, nrhs, a, lda, b, ldb, x, ldx, workspace, workspace_size, stream);
+}
+template <> void triangular_solver<float, linalg_backend::ROCM_NATIVE>::apply(bool upper, bool transpose,
+    bool unit_diagonal, linalg_int n, linalg_int nrhs, const float* a, linalg_int lda, const float* b,
+    linalg_int ldb, float* x, linalg_int ldx, void* workspace, size_t workspace_size, void* stream)
+{
+    tr_apply<float>(upper, transpose, unit_diagonal, n, nrhs, a, lda, b, ldb, x, ldx, workspace, workspace_size, stream);
+}
+template <> size_t triangular_solver<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int n,
+    linalg_int nrhs)
+{
+    return tr_ws<double>(n, nrhs);
+}
+template <> size_t triangular_solver<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int n,
+    linalg_int nrhs)
+{
+    return tr_ws<float>(n, nrhs);
 }
 
 namespace rocm_native

@@ -91,6 +91,14 @@ static void host_checks()
     EXPECT(throws<std::invalid_argument>(
         [&] { SV::apply(m, piv, 4, m, linear_solver_type::CHOLESKY_LINEAR_SOLVER, ws, sizeof(ws)); }));
     EXPECT(throws<std::invalid_argument>([&] { SV::workspace_size(4, linear_solver_type::CHOLESKY_LINEAR_SOLVER); }));
+    // the triangular solver's argument checks (dimensions, aliasing, strides) run on the host
+    using TS = linalg::triangular_solver<double, linalg::linalg_backend::ROCM_NATIVE>;
+    double tb[16], tx[16];
+    EXPECT(TS::workspace_size(70, 3) > 0);
+    EXPECT(throws<std::invalid_argument>([&] { TS::workspace_size(0, 2); }));
+    EXPECT(throws<std::invalid_argument>([&] { TS::workspace_size(2, -1); }));
+    EXPECT(throws<std::invalid_argument>([&] { TS::apply(true, false, false, 4, 2, m, 4, tb, 2, tb, 2, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { TS::apply(true, false, false, 4, 2, m, 3, tb, 2, tx, 2, ws, sizeof(ws)); }));
 }
 
 #define HIP_OK(x) EXPECT((x) == hipSuccess)
@@ -201,6 +209,49 @@ static void gpu_checks()
     (void)hipFree(db);
     (void)hipFree(dxm);
     (void)hipFree(dwm);
+
+    // triangular_solver: upper, non-unit, with a known solution (padded row strides)
+    {
+        using TS = linalg::triangular_solver<double, linalg::linalg_backend::ROCM_NATIVE>;
+        const long long nt = 70, nr = 3, lda = nt + 2, ldb = nr + 1, ldx = nr + 3;
+        std::vector<double> at(nt * lda, 0.0), bt(nt * ldb, -123.0), xt(nt * ldx, -123.0), want(nt * nr);
+        for (long long i = 0; i < nt; ++i) {
+            for (long long j = i; j < nt; ++j) at[i * lda + j] = i == j ? 2.0 + 0.02 * i : 0.01 * std::sin(double(i * nt + j + 1));
+            for (long long j = 0; j < nr; ++j) want[i * nr + j] = std::sin(double(i * nr + j + 1));
+        }
+        for (long long i = 0; i < nt; ++i)
+            for (long long j = 0; j < nr; ++j) {
+                double s = 0.0;
+                for (long long k = i; k < nt; ++k) s += at[i * lda + k] * want[k * nr + j];
+                bt[i * ldb + j] = s;
+            }
+        double *d_a = nullptr, *d_b = nullptr, *d_x = nullptr;
+        void* d_w = nullptr;
+        const size_t wst = TS::workspace_size(nt, nr);
+        HIP_OK(hipMalloc(&d_a, at.size() * sizeof(double)));
+        HIP_OK(hipMalloc(&d_b, bt.size() * sizeof(double)));
+        HIP_OK(hipMalloc(&d_x, xt.size() * sizeof(double)));
+        HIP_OK(hipMalloc(&d_w, wst));
+        HIP_OK(hipMemcpy(d_a, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d_b, bt.data(), bt.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d_x, xt.data(), xt.size() * sizeof(double), hipMemcpyHostToDevice));
+        TS::apply(true, false, false, nt, nr, d_a, lda, d_b, ldb, d_x, ldx, d_w, wst, nullptr);
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemcpy(xt.data(), d_x, xt.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double err = 0.0;
+        bool pad = true;
+        for (long long i = 0; i < nt; ++i) {
+            for (long long j = 0; j < nr; ++j) err = std::fmax(err, std::fabs(xt[i * ldx + j] - want[i * nr + j]));
+            for (long long j = nr; j < ldx; ++j) pad = pad && xt[i * ldx + j] == -123.0;
+        }
+        std::printf("triangular_solver n=%lld nrhs=%lld upper: max error %.2e\n", nt, nr, err);
+        EXPECT(err <= 1e-13);
+        EXPECT(pad);
+        (void)hipFree(d_a);
+        (void)hipFree(d_b);
+        (void)hipFree(d_x);
+        (void)hipFree(d_w);
+    }
 }
 
 int main(int argc, char** argv)
