@@ -34,6 +34,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -79,6 +80,8 @@ def parse():
     p.add_argument("--no-configs", action="store_true", help="skip the configuration grids")
     p.add_argument("--config-sizes", default="4096,8192,16384",
                    help="orders of the plain FP64 grid (nrhs 1 and 64, A iid N(0,1) and A + n I)")
+    p.add_argument("--consumer-sizes", default="1024,4096",
+                   help="orders of the determinant / inversion timing (comma-separated)")
     p.add_argument("--mixed-kappas", default="1e2,1e10",
                    help="further condition numbers of the mixed grid at the headline order")
     p.add_argument("--mixed-extra-n", type=int, default=0,
@@ -318,6 +321,58 @@ def triangular_bench(torch, n, seed, dev, reps=3):
         del b, x, ws
     del a
     return {"n": n, "cases": out}
+
+
+def consumers_bench(torch, sizes, seed, dev, reps=2):
+    """determinant<double, ROCM_NATIVE> and inversion<double, ROCM_NATIVE> (LU mode, pivots in
+    the workspace) on N(0,1) matrices of each order: median of `reps` HIP-event-timed calls after
+    a warmup (the factorisation included), the determinant's ratio to the product of the factor
+    diagonal computed by torch from the same factors, and max|A^-1 A - I| of the inverse."""
+    import linearalgebra_amd as la
+    LU = la.linear_solver_type.LU_LINEAR_SOLVER
+    out = []
+    for n in sizes:
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed + n)
+        # scale sqrt(e/n): E log|det| of an N(0,1) matrix is about (n ln n - n)/2, so the
+        # determinant stays within double range
+        a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g) * (2.718281828459045 / n) ** 0.5
+        m = torch.empty_like(a)
+        row = {"n": n}
+        for name, op in (("determinant", la.determinant), ("inversion", la.inversion)):
+            ws = torch.empty(op.workspace_size(n, LU), dtype=torch.uint8, device=dev)
+            times = []
+            val = None
+            for r in range(reps + 1):
+                m.copy_(a)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                val = op.apply(m, None, n, LU, ws)
+                e1.record()
+                torch.cuda.synchronize()
+                if r > 0:
+                    times.append(e0.elapsed_time(e1))
+            ms = sorted(times)[len(times) // 2]
+            row[f"{name}_ms"] = round(ms, 3)
+            if name == "determinant":
+                sign, logabs = torch.linalg.slogdet(a)
+                row["determinant"] = val
+                row["log_abs_det"] = math.log(abs(val)) if val != 0 and math.isfinite(val) else None
+                row["log_abs_det_torch"] = float(logabs)
+                row["sign_matches_torch"] = bool((val > 0) == (float(sign) > 0))
+            else:
+                err = float((m @ a - torch.eye(n, dtype=a.dtype, device=dev)).abs().max())
+                row["inverse_max_err"] = err
+                flops = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 3
+                row["inversion_tflops"] = round(flops / (ms * 1e-3) / 1e12, 2)
+            del ws
+        log(f"consumers n={n}: determinant {row['determinant_ms']:.1f} ms, inversion {row['inversion_ms']:.1f} ms "
+            f"({row['inversion_tflops']} TFLOP/s), max|inv A - I| {row['inverse_max_err']:.1e}")
+        out.append(row)
+        del a, m
+    return out
 
 
 # ----------------------------------------------------------------------------- batched
@@ -576,12 +631,14 @@ def main():
     plain_configs = None
     mixed_configs = None
     triangular = None
+    consumers = None
     if not args.no_configs:
         plain_configs = plain_configs_bench(torch, [int(v) for v in args.config_sizes.split(",") if v],
                                             (1, 64), args.seed, dev)
         kappas = [float(v) for v in args.mixed_kappas.split(",") if v]
         mixed_configs = mixed_configs_bench(torch, [n], kappas, args.seed, dev, args.restart) if kappas else []
         triangular = triangular_bench(torch, n, args.seed, dev)
+        consumers = consumers_bench(torch, [int(v) for v in args.consumer_sizes.split(",") if v], args.seed, dev)
         if args.mixed_extra_n > 0:
             mixed_configs += mixed_configs_bench(torch, [args.mixed_extra_n], [1e2, args.kappa] + [k for k in kappas if k > 1e6],
                                                  args.seed, dev, args.restart)
@@ -638,6 +695,7 @@ def main():
         "plain_configs": plain_configs,
         "mixed_configs": mixed_configs,
         "triangular_solver": triangular,
+        "determinant_inversion": consumers,
         "batched": batched,
         "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,

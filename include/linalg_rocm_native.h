@@ -99,6 +99,40 @@ LRN_API int lrn_triangular_solver_f32(int upper, int transpose, int unit, int64_
     int64_t lda, const float* b, int64_t ldb, float* x, int64_t ldx, void* workspace, size_t workspace_size,
     void* stream);
 
+/* ---- determinant<T, ROCM_NATIVE> -------------------------------------------------------
+ * Replaces determinant<T, B>::apply / workspace_size (include/matrix_operation/determinant.h:
+ * 13-55; ROCM_BLAS version src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58,440-482).
+ * LU mode factors m in place (as lu_decomposition; pivots to `pivot`, or to the workspace
+ * when pivot is NULL, since the GPU reference ignores that argument); LU_UPFRONT reads the
+ * caller's factors and pivots (pivot required). *det = product of U's diagonal in increasing
+ * row order, negated for an odd number of interchanges, 0 when the factorisation finds a
+ * zero pivot: the GPU reference's result. The CPU reference (src/matrix_operation/cpu/
+ * determinant.cpp:19-28) omits the interchange sign (determinant.h:35-36), so the two agree
+ * up to sign. Argument errors: -1 m, -2 pivot, -3 lda, -4 type, -5 workspace, -7 det.
+ * Cholesky modes return LRN_E_UNSUPPORTED. SYNCHRONOUS on stream (the value is returned on
+ * the host, as the reference's apply returns it). */
+LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_determinant_workspace_size_f32(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_determinant_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, double* det, void* stream);
+LRN_API int lrn_determinant_f32(float* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, float* det, void* stream);
+
+/* ---- inversion<T, ROCM_NATIVE> ---------------------------------------------------------
+ * Replaces inversion<T, B>::apply / workspace_size (include/matrix_operation/inversion.h:
+ * 13-53; CPU version src/matrix_operation/cpu/native/inversion_native.cpp:229-304,411-436).
+ * m (row-major lda x lda) is overwritten with A^-1. LU mode factors first (pivots as for
+ * the determinant) and, on a zero pivot, leaves the factors in m; LU_UPFRONT inverts the
+ * caller's factors and pivots. Argument errors: -1 m, -2 pivot, -3 lda, -4 type,
+ * -5 workspace. Cholesky modes return LRN_E_UNSUPPORTED. Workspace holds one lda x lda
+ * matrix besides the factorisation's. SYNCHRONOUS on stream (it reports the panel's status). */
+LRN_API int lrn_inversion_workspace_size_f64(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_inversion_workspace_size_f32(int64_t lda, int type, size_t* bytes);
+LRN_API int lrn_inversion_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, void* stream);
+LRN_API int lrn_inversion_f32(float* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, void* stream);
+
 /* ---- multi-RHS extension (no reference counterpart: the reference solves one RHS;
  * docs/backend_roadmap.md:285 lists multi-RHS as future work) ---------------------------
  * Same semantics as linear_solver with B a row-major lda x nrhs block (row stride ldb >=

@@ -1,6 +1,7 @@
 /*
  * C++ face of the MI355X-native LU solve path: the reference's operator templates
- * (lu_decomposition<T, B>, linear_solver<T, B>, triangular_solver<T, B>) specialised for a new backend tag
+ * (lu_decomposition<T, B>, linear_solver<T, B>, triangular_solver<T, B>, determinant<T, B>,
+ * inversion<T, B>) specialised for a new backend tag
  * linalg_backend::ROCM_NATIVE. Same names, argument meaning, ownership and error
  * behaviour as the reference; the bodies (csrc/rocm_native_backend.cpp) call the C ABI of
  * linalg_rocm_native.h.
@@ -11,6 +12,8 @@
  *   include/matrix_operation/lu_decomposition.h:15-54  lu_decomposition<T, B>
  *   include/matrix_operation/linear_solver.h:15-65     linear_solver_type, linear_solver<T, B>
  *   include/matrix_operation/triangular_solver.h:20-38 triangular_solver<T, B>
+ *   include/matrix_operation/determinant.h:13-55       determinant<T, B>
+ *   include/matrix_operation/inversion.h:13-53         inversion<T, B>
  *
  * Inside the reference tree, define LRN_WITH_REFERENCE_HEADERS and put the reference's
  * include root on the include path: the reference's own declarations are used and only the
@@ -33,6 +36,8 @@
 #include "include/matrix_operation/linear_solver.h"
 #include "include/matrix_operation/lu_decomposition.h"
 #include "include/matrix_operation/triangular_solver.h"
+#include "include/matrix_operation/determinant.h"
+#include "include/matrix_operation/inversion.h"
 #else
 
 #if !defined(linalg_int)
@@ -82,6 +87,22 @@ template <typename T, linalg_backend Backend> struct triangular_solver
         void* stream = nullptr);
     static size_t workspace_size(linalg_int n, linalg_int nrhs);
 };
+
+// include/matrix_operation/determinant.h:13-55
+template <typename T, linalg_backend Backend> struct determinant
+{
+    static T apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type, void* workspace,
+        size_t workspace_size, void* stream = nullptr);
+    static size_t workspace_size(linalg_int lda, linear_solver_type type);
+};
+
+// include/matrix_operation/inversion.h:13-53
+template <typename T, linalg_backend Backend> struct inversion
+{
+    static void apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type, void* workspace,
+        size_t workspace_size, void* stream = nullptr);
+    static size_t workspace_size(linalg_int lda, linear_solver_type type);
+};
 }  // namespace linalg
 #endif
 
@@ -117,6 +138,25 @@ template <> size_t triangular_solver<double, linalg_backend::ROCM_NATIVE>::works
     linalg_int nrhs);
 template <> size_t triangular_solver<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int n,
     linalg_int nrhs);
+
+// determinant (signed, as the GPU reference's) and in-place inversion; m, pivot and workspace
+// are device memory (pivot may be null in LU mode); both synchronous on `stream`.
+template <> double determinant<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream);
+template <> float determinant<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream);
+template <> size_t determinant<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type);
+template <> size_t determinant<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type);
+template <> void inversion<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream);
+template <> void inversion<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream);
+template <> size_t inversion<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type);
+template <> size_t inversion<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type);
 
 // Extensions of this backend (no reference counterpart).
 namespace rocm_native

@@ -15,6 +15,7 @@ from .lu import (  # noqa: F401
 
 from .mixed import gesv_mixed  # noqa: F401,E402
 from .triangular import triangular_solver  # noqa: F401,E402
+from .inverse import determinant, inversion  # noqa: F401,E402
 
-__all__ = ["gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type", "triangular_solver", "LinalgError",
-           "set_tuning", "get_tuning"]
+__all__ = ["determinant", "inversion", "gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type",
+           "triangular_solver", "LinalgError", "set_tuning", "get_tuning"]

@@ -31,6 +31,14 @@ SIGNATURES = {
     "lrn_triangular_solver_workspace_size_f32": [_i64, _i64, ctypes.POINTER(_sz)],
     "lrn_triangular_solver_f64": [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp],
     "lrn_triangular_solver_f32": [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp],
+    "lrn_determinant_workspace_size_f64": [_i64, _int, ctypes.POINTER(_sz)],
+    "lrn_determinant_workspace_size_f32": [_i64, _int, ctypes.POINTER(_sz)],
+    "lrn_determinant_f64": [_vp, _vp, _i64, _int, _vp, _sz, ctypes.POINTER(ctypes.c_double), _vp],
+    "lrn_determinant_f32": [_vp, _vp, _i64, _int, _vp, _sz, ctypes.POINTER(ctypes.c_float), _vp],
+    "lrn_inversion_workspace_size_f64": [_i64, _int, ctypes.POINTER(_sz)],
+    "lrn_inversion_workspace_size_f32": [_i64, _int, ctypes.POINTER(_sz)],
+    "lrn_inversion_f64": [_vp, _vp, _i64, _int, _vp, _sz, _vp],
+    "lrn_inversion_f32": [_vp, _vp, _i64, _int, _vp, _sz, _vp],
     "lrn_linear_solver_multi_workspace_size_f64": [_i64, _i64, _int, ctypes.POINTER(_sz)],
     "lrn_linear_solver_multi_workspace_size_f32": [_i64, _i64, _int, ctypes.POINTER(_sz)],
     "lrn_linear_solver_multi_f64": [_vp, _vp, _i64, _vp, _i64, _i64, _int, _vp, _sz, _vp],

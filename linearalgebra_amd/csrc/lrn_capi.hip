@@ -37,6 +37,7 @@
 #include "lrn_trace.hpp"
 #include "lrn_batched.hpp"
 #include "lrn_calu.hpp"
+#include "lrn_inverse.hpp"
 
 namespace lrn {
 namespace {
@@ -1143,6 +1144,134 @@ int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t l
     return getrs_run<T>(m, lda, reinterpret_cast<const long long*>(pivot), b, nrhs, ldb, G, skip, s);
 }
 
+// ------------------------------------------------------------- determinant and inversion
+// Direct consumers of the factors (include/matrix_operation/determinant.h:13-55,
+// include/matrix_operation/inversion.h:13-53). LU mode factors m in place first, into the
+// caller's pivots or, when pivot is null (the GPU reference ignores that argument,
+// src/matrix_operation/gpu/hipblas/inversion_hip.hip:662-700), into pivots carved from the
+// workspace; LU_UPFRONT reads the caller's factors and pivots.
+bool is_lu_type(int type) { return type == LRN_LU_LINEAR_SOLVER || type == LRN_LU_UPFRONT_LINEAR_SOLVER; }
+
+template <typename T> struct ConsumerLayout {
+    GetrfLayout<T> F;
+    long long* piv = nullptr;
+    int* info = nullptr;
+    T* result = nullptr;
+};
+
+template <typename T> ConsumerLayout<T> carve_consumer(Arena& ar, long long n, int type)
+{
+    ConsumerLayout<T> L;
+    if (type == LRN_LU_LINEAR_SOLVER) {
+        L.F = carve_getrf<T>(ar);
+        L.piv = ar.next<long long>(static_cast<size_t>(n));
+    }
+    L.info = ar.next<int>(64);
+    L.result = ar.next<T>(1);
+    return L;
+}
+
+template <typename T> size_t det_ws_bytes(long long n, int type)
+{
+    Arena ar;
+    carve_consumer<T>(ar, n, type);
+    return ar.off;
+}
+
+template <typename T> size_t inverse_ws_bytes(long long n, int type)
+{
+    Arena ar;
+    carve_consumer<T>(ar, n, type);
+    carve_getrs<T>(ar, n, n);
+    ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(n));
+    return ar.off;
+}
+
+// Factor (LU mode) and return the pivots to use; info ends up in L.info on the device.
+template <typename T>
+int consumer_factor(T* m, int64_t* pivot, long long n, int type, const ConsumerLayout<T>& L, hipStream_t s,
+    const long long** piv_out)
+{
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.info), 0u, 64, s));
+    if (type == LRN_LU_LINEAR_SOLVER) {
+        long long* piv = pivot != nullptr ? reinterpret_cast<long long*>(pivot) : L.piv;
+        LRN_TRY(getrf_dispatch<T>(m, n, piv, L.F.ws, s));
+        LRN_HIP(hipMemcpyAsync(L.info, L.F.ws.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+        *piv_out = piv;
+    } else {
+        *piv_out = reinterpret_cast<const long long*>(pivot);
+    }
+    return LRN_OK;
+}
+
+// Status of the cooperative panel's bounded waits (LU mode), read after a synchronisation.
+template <typename T> int consumer_status(const ConsumerLayout<T>& L, int type, hipStream_t s)
+{
+    if (type != LRN_LU_LINEAR_SOLVER) return LRN_OK;
+    int err = 0;
+    LRN_HIP(hipMemcpyAsync(&err, L.F.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipStreamSynchronize(s));
+    return err != 0 ? LRN_E_TIMEOUT : LRN_OK;
+}
+
+template <typename T>
+int determinant_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace, size_t ws_size, T* det,
+    void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -3;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!is_lu_type(type)) return -4;
+    if (type == LRN_LU_UPFRONT_LINEAR_SOLVER && pivot == nullptr) return -2;
+    if (det == nullptr) return -7;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < det_ws_bytes<T>(lda, type)) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const auto L = carve_consumer<T>(ar, lda, type);
+    const long long* piv = nullptr;
+    LRN_TRY(consumer_factor<T>(m, pivot, lda, type, L, s, &piv));
+    hipLaunchKernelGGL(lu_det_kernel<T>, dim3(1), dim3(256), 0, s, m, static_cast<long long>(lda),
+        static_cast<long long>(lda), piv, L.info, L.result);
+    LRN_HIP(hipGetLastError());
+    T host = T(0);
+    LRN_HIP(hipMemcpyAsync(&host, L.result, sizeof(T), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipStreamSynchronize(s));
+    LRN_TRY(consumer_status<T>(L, type, s));
+    *det = host;
+    return LRN_OK;
+}
+
+// A^-1 in place: X = I, LU X = P X with the substitution kernels of the multi-RHS solve, then
+// X is copied over m unless the factorisation found a zero pivot (m then keeps the factors,
+// src/matrix_operation/cpu/native/inversion_native.cpp:421-436).
+template <typename T>
+int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace, size_t ws_size, void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -3;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!is_lu_type(type)) return -4;
+    if (type == LRN_LU_UPFRONT_LINEAR_SOLVER && pivot == nullptr) return -2;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < inverse_ws_bytes<T>(lda, type)) return -5;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long n = lda;
+    Arena ar(workspace, ws_size);
+    const auto L = carve_consumer<T>(ar, n, type);
+    const auto G = carve_getrs<T>(ar, n, n);
+    T* X = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(n));
+    const long long* piv = nullptr;
+    LRN_TRY(consumer_factor<T>(m, pivot, n, type, L, s, &piv));
+    const unsigned grid = static_cast<unsigned>(std::min<long long>((n * n + 255) / 256, 1ll << 16));
+    hipLaunchKernelGGL(eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n);
+    LRN_HIP(hipGetLastError());
+    LRN_TRY(getrs_run<T>(m, n, piv, X, n, n, G, L.info, s));
+    hipLaunchKernelGGL(copy_unless_kernel<T>, dim3(grid), dim3(256), 0, s, X, m, n * n, L.info);
+    LRN_HIP(hipGetLastError());
+    return consumer_status<T>(L, type, s);
+}
+
 template <typename T>
 int debug_panel_impl(T* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* ws, size_t ws_size, void* stream)
 {
@@ -1701,6 +1830,64 @@ LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, f
     int type, void* workspace, size_t workspace_size, void* stream)
 {
     return solver_impl<float>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!lrn::is_lu_type(type)) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::det_ws_bytes<double>(lda, type);
+    return LRN_OK;
+}
+LRN_API int lrn_determinant_workspace_size_f32(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!lrn::is_lu_type(type)) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::det_ws_bytes<float>(lda, type);
+    return LRN_OK;
+}
+LRN_API int lrn_determinant_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, double* det, void* stream)
+{
+    return lrn::determinant_impl<double>(m, pivot, lda, type, workspace, workspace_size, det, stream);
+}
+LRN_API int lrn_determinant_f32(float* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, float* det, void* stream)
+{
+    return lrn::determinant_impl<float>(m, pivot, lda, type, workspace, workspace_size, det, stream);
+}
+
+LRN_API int lrn_inversion_workspace_size_f64(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!lrn::is_lu_type(type)) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::inverse_ws_bytes<double>(lda, type);
+    return LRN_OK;
+}
+LRN_API int lrn_inversion_workspace_size_f32(int64_t lda, int type, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
+    if (!lrn::is_lu_type(type)) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::inverse_ws_bytes<float>(lda, type);
+    return LRN_OK;
+}
+LRN_API int lrn_inversion_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return lrn::inversion_impl<double>(m, pivot, lda, type, workspace, workspace_size, stream);
+}
+LRN_API int lrn_inversion_f32(float* m, int64_t* pivot, int64_t lda, int type, void* workspace,
+    size_t workspace_size, void* stream)
+{
+    return lrn::inversion_impl<float>(m, pivot, lda, type, workspace, workspace_size, stream);
 }
 
 LRN_API int lrn_getrs_f32f64_workspace_size(int64_t n, size_t* bytes)

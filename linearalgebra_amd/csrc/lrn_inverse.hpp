@@ -1,0 +1,71 @@
+// Kernels of the direct consumers of the LU factors: the determinant (diagonal product and
+// interchange sign) and the explicit inverse (identity right-hand sides, conditional copy-back).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lrn {
+
+// det(A) from packed L\U factors: the diagonal of U multiplied in increasing row order with
+// one running product (the rounding of the reference's sequential loops,
+// src/matrix_operation/cpu/determinant.cpp:19-28 and
+// src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58), negated when the pivots hold an
+// odd number of interchanges (piv[i] != i + 1, 1-based), and zero when `info` reports a zero
+// pivot. One workgroup: 256 diagonal entries are staged through LDS per step and one lane
+// keeps the running product. piv may be null (no sign), info may be null (not checked).
+template <typename T>
+__global__ __launch_bounds__(256) void lu_det_kernel(const T* __restrict__ LU, long long lda, long long n,
+    const long long* __restrict__ piv, const int* __restrict__ info, T* __restrict__ out)
+{
+    __shared__ T diag[256];
+    __shared__ int odd;
+    const int t = threadIdx.x;
+    if (t == 0) odd = 0;
+    T prod = T(1);
+    int mine = 0;
+    for (long long i0 = 0; i0 < n; i0 += 256) {
+        const long long i = i0 + t;
+        __syncthreads();
+        if (i < n) {
+            diag[t] = LU[i * lda + i];
+            if (piv != nullptr && piv[i] != i + 1) mine ^= 1;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const int cnt = static_cast<int>(n - i0 < 256 ? n - i0 : 256);
+            for (int k = 0; k < cnt; ++k) prod *= diag[k];
+        }
+    }
+    if (mine) atomicXor(&odd, 1);
+    __syncthreads();
+    if (t == 0) {
+        T r = odd ? -prod : prod;
+        if (info != nullptr && *info != 0) r = T(0);
+        out[0] = r;
+    }
+}
+
+// X = I (n x n, row stride ldx); grid-stride loop.
+template <typename T>
+__global__ __launch_bounds__(256) void eye_kernel(T* __restrict__ X, long long n, long long ldx)
+{
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; idx < n * n; idx += stride) {
+        const long long i = idx / n, j = idx - i * n;
+        X[i * ldx + j] = (i == j) ? T(1) : T(0);
+    }
+}
+
+// dst = src (n x n, both with row stride n) unless *skip is nonzero (a singular factorisation
+// leaves the factors in place, as the reference's LU inversion does); grid-stride loop.
+template <typename T>
+__global__ __launch_bounds__(256) void copy_unless_kernel(const T* __restrict__ src, T* __restrict__ dst,
+    long long count, const int* __restrict__ skip)
+{
+    if (skip != nullptr && *skip != 0) return;
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; idx < count; idx += stride)
+        dst[idx] = src[idx];
+}
+
+}  // namespace lrn

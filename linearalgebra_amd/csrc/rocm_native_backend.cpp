@@ -44,6 +44,10 @@ template <> struct ops<double> {
     static constexpr auto svm = lrn_linear_solver_multi_f64;
     static constexpr auto tr_ws = lrn_triangular_solver_workspace_size_f64;
     static constexpr auto tr = lrn_triangular_solver_f64;
+    static constexpr auto det_ws = lrn_determinant_workspace_size_f64;
+    static constexpr auto det = lrn_determinant_f64;
+    static constexpr auto inv_ws = lrn_inversion_workspace_size_f64;
+    static constexpr auto inv = lrn_inversion_f64;
 };
 template <> struct ops<float> {
     static constexpr auto lu_ws = lrn_lu_decomposition_workspace_size_f32;
@@ -54,6 +58,10 @@ template <> struct ops<float> {
     static constexpr auto svm = lrn_linear_solver_multi_f32;
     static constexpr auto tr_ws = lrn_triangular_solver_workspace_size_f32;
     static constexpr auto tr = lrn_triangular_solver_f32;
+    static constexpr auto det_ws = lrn_determinant_workspace_size_f32;
+    static constexpr auto det = lrn_determinant_f32;
+    static constexpr auto inv_ws = lrn_inversion_workspace_size_f32;
+    static constexpr auto inv = lrn_inversion_f32;
 };
 
 template <typename T> bool lu_apply(T* m, linalg_int lda, linalg_int* pivot, void* ws, size_t ws_size, int* info, void* stream)
@@ -171,6 +179,78 @@ template <> size_t triangular_solver<float, linalg_backend::ROCM_NATIVE>::worksp
     linalg_int nrhs)
 {
     return tr_ws<float>(n, nrhs);
+}
+
+template <typename T>
+T det_apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type, void* ws, size_t ws_size, void* stream)
+{
+    if (m == nullptr) throw std::invalid_argument("determinant<ROCM_NATIVE>: m must not be null");
+    if (lda <= 0) throw std::invalid_argument("determinant<ROCM_NATIVE>: lda must be positive");
+    T out = T(0);
+    check(ops<T>::det(m, reinterpret_cast<int64_t*>(pivot), lda, solver_type(type), ws, ws_size, &out, stream),
+        "determinant<ROCM_NATIVE>::apply");
+    return out;
+}
+template <typename T> size_t det_ws(linalg_int lda, linear_solver_type type)
+{
+    size_t bytes = 0;
+    check(ops<T>::det_ws(lda, solver_type(type), &bytes), "determinant<ROCM_NATIVE>::workspace_size");
+    return bytes;
+}
+template <typename T>
+void inv_apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type, void* ws, size_t ws_size, void* stream)
+{
+    if (m == nullptr) throw std::invalid_argument("inversion<ROCM_NATIVE>: m must not be null");
+    if (lda <= 0) throw std::invalid_argument("inversion<ROCM_NATIVE>: lda must be positive");
+    check(ops<T>::inv(m, reinterpret_cast<int64_t*>(pivot), lda, solver_type(type), ws, ws_size, stream),
+        "inversion<ROCM_NATIVE>::apply");
+}
+template <typename T> size_t inv_ws(linalg_int lda, linear_solver_type type)
+{
+    size_t bytes = 0;
+    check(ops<T>::inv_ws(lda, solver_type(type), &bytes), "inversion<ROCM_NATIVE>::workspace_size");
+    return bytes;
+}
+
+template <> double determinant<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    return det_apply<double>(m, pivot, lda, type, workspace, workspace_size, stream);
+}
+template <> float determinant<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    return det_apply<float>(m, pivot, lda, type, workspace, workspace_size, stream);
+}
+template <> size_t determinant<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return det_ws<double>(lda, type);
+}
+template <> size_t determinant<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return det_ws<float>(lda, type);
+}
+template <> void inversion<double, linalg_backend::ROCM_NATIVE>::apply(double* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    inv_apply<double>(m, pivot, lda, type, workspace, workspace_size, stream);
+}
+template <> void inversion<float, linalg_backend::ROCM_NATIVE>::apply(float* m, linalg_int* pivot,
+    linalg_int lda, linear_solver_type type, void* workspace, size_t workspace_size, void* stream)
+{
+    inv_apply<float>(m, pivot, lda, type, workspace, workspace_size, stream);
+}
+template <> size_t inversion<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return inv_ws<double>(lda, type);
+}
+template <> size_t inversion<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    linear_solver_type type)
+{
+    return inv_ws<float>(lda, type);
 }
 
 namespace rocm_native

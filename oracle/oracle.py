@@ -11,6 +11,12 @@ Reference correspondence:
              restated in numpy (scalar-path order); pinned by the reference's known-answer
              generator (Testing/Cxx/triangular_solver_test_cases.h:12-68, restated as
              trsm_known_answer_cases) and scipy's solve_triangular
+  determinant -> determinant<T, B>::apply, LU modes (src/matrix_operation/cpu/determinant.cpp:19-28
+             unsigned; src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58 signed); pinned
+             by the reference's known answers (Testing/Cxx/TestDeterminant.cxx:82-126,
+             Testing/Cxx/TestGpuMatrixOperations.cxx:1466-1492) and numpy.linalg.det
+  inverse_from_factors -> lu_invert (src/matrix_operation/cpu/native/inversion_native.cpp:229-304);
+             pinned by A A^-1 = I on the reference's inversion input and numpy.linalg.inv
 """
 from __future__ import annotations
 
@@ -340,3 +346,61 @@ def trsm_known_answer_cases(dtype=np.float64):
                                 acc = T(acc + T(c * expected[k, j]))
                             b[i, j] = acc
                     yield upper, transpose, unit, n, nrhs, lda, ldb, ldx, a, b, expected
+
+
+# ------------------------------------------------------------------- determinant and inverse
+
+def determinant(lu: np.ndarray, piv: np.ndarray | None, info: int = 0, signed: bool = True):
+    """det from packed L\\U factors: U's diagonal multiplied in increasing row order with one
+    running product in the factors' type (src/matrix_operation/cpu/determinant.cpp:19-28 starts
+    from m[0]; src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58 from 1 * m[0], the same
+    value), 0 when the factorisation reported a zero pivot (info != 0). signed=True applies the
+    interchange sign as the GPU reference does (one flip per piv[i] != i + 1); signed=False is
+    the CPU reference's unsigned product."""
+    n = lu.shape[0]
+    T = lu.dtype.type
+    if info != 0:
+        return T(0)
+    det = T(1)
+    for i in range(n):
+        det = T(det * lu[i, i])
+    if signed and piv is not None:
+        odd = int(np.count_nonzero(np.asarray(piv[:n]) != np.arange(1, n + 1))) & 1
+        if odd:
+            det = T(-det)
+    return det
+
+
+def inverse_from_factors(lu: np.ndarray, piv: np.ndarray) -> np.ndarray:
+    """A^-1 from packed factors the way lu_invert does it
+    (src/matrix_operation/cpu/native/inversion_native.cpp:229-304): X = I with rows swapped in
+    pivot order, then row i -= l_ik * row k for k < i (forward), row i -= u_ik * row k for k > i
+    and row i *= 1 / u_ii (backward, i decreasing), every update a whole row."""
+    n = lu.shape[0]
+    T = lu.dtype.type
+    x = np.eye(n, dtype=lu.dtype)
+    for i in range(n):
+        p = int(piv[i]) - 1
+        if p != i:
+            x[[i, p]] = x[[p, i]]
+    for i in range(n):
+        for k in range(i):
+            x[i] = x[i] - lu[i, k] * x[k]
+    for i in range(n - 1, -1, -1):
+        for k in range(i + 1, n):
+            x[i] = x[i] - lu[i, k] * x[k]
+        x[i] = x[i] * (T(1) / lu[i, i])
+    return x
+
+
+def inversion_spd_case(dim: int, dtype=np.float64, seed: int = 0) -> np.ndarray:
+    """The reference suite's inversion input (Testing/Cxx/TestInversion.cxx:44-64): A = R R^T with
+    R unit lower triangular and R[i, j] = 0.4 * u (u uniform in [0, 1]) below the diagonal
+    (seeded numpy draws stand in for rand())."""
+    rng = np.random.default_rng(seed)
+    r = np.eye(dim)
+    for i in range(dim):
+        for j in range(i):
+            r[i, j] = 0.4 * rng.random()
+    r = r.astype(dtype)
+    return (r @ r.T).astype(dtype)

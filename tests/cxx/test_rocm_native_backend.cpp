@@ -8,6 +8,7 @@
 //                                        relative), the zero-pivot KAT and a mixed solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -99,6 +100,17 @@ static void host_checks()
     EXPECT(throws<std::invalid_argument>([&] { TS::workspace_size(2, -1); }));
     EXPECT(throws<std::invalid_argument>([&] { TS::apply(true, false, false, 4, 2, m, 4, tb, 2, tb, 2, ws, sizeof(ws)); }));
     EXPECT(throws<std::invalid_argument>([&] { TS::apply(true, false, false, 4, 2, m, 3, tb, 2, tx, 2, ws, sizeof(ws)); }));
+    // determinant / inversion: argument checks on the host, Cholesky modes not provided
+    using DT = linalg::determinant<double, linalg::linalg_backend::ROCM_NATIVE>;
+    using IV = linalg::inversion<double, linalg::linalg_backend::ROCM_NATIVE>;
+    EXPECT(DT::workspace_size(100, linear_solver_type::LU_LINEAR_SOLVER) >
+           DT::workspace_size(100, linear_solver_type::LU_UPFRONT_LINEAR_SOLVER));
+    EXPECT(IV::workspace_size(100, linear_solver_type::LU_LINEAR_SOLVER) >= 100 * 100 * sizeof(double));
+    EXPECT(throws<std::invalid_argument>([&] { DT::apply(nullptr, nullptr, 4, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { DT::apply(m, nullptr, 0, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { DT::workspace_size(4, linear_solver_type::CHOLESKY_LINEAR_SOLVER); }));
+    EXPECT(throws<std::invalid_argument>([&] { IV::apply(nullptr, nullptr, 4, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
+    EXPECT(throws<std::invalid_argument>([&] { IV::apply(m, nullptr, 4, static_cast<linear_solver_type>(5), ws, sizeof(ws)); }));
 }
 
 #define HIP_OK(x) EXPECT((x) == hipSuccess)
@@ -250,6 +262,35 @@ static void gpu_checks()
         (void)hipFree(d_a);
         (void)hipFree(d_b);
         (void)hipFree(d_x);
+        (void)hipFree(d_w);
+    }
+
+    // determinant and inversion of the reference's GPU case (TestGpuMatrixOperations.cxx:1466-1492):
+    // A = diag(4, 2) gives det 8 and inverse diag(0.25, 0.5); a row swap flips the sign
+    {
+        using DT = linalg::determinant<double, linalg::linalg_backend::ROCM_NATIVE>;
+        using IV = linalg::inversion<double, linalg::linalg_backend::ROCM_NATIVE>;
+        const double diag[4] = {4.0, 0.0, 0.0, 2.0}, swapped[4] = {0.0, 2.0, 4.0, 0.0};
+        double* d_m = nullptr;
+        void* d_w = nullptr;
+        const size_t wsd = std::max(DT::workspace_size(2, linear_solver_type::LU_LINEAR_SOLVER),
+            IV::workspace_size(2, linear_solver_type::LU_LINEAR_SOLVER));
+        HIP_OK(hipMalloc(&d_m, sizeof(diag)));
+        HIP_OK(hipMalloc(&d_w, wsd));
+        HIP_OK(hipMemcpy(d_m, diag, sizeof(diag), hipMemcpyHostToDevice));
+        const double det = DT::apply(d_m, nullptr, 2, linear_solver_type::LU_LINEAR_SOLVER, d_w, wsd);
+        HIP_OK(hipMemcpy(d_m, swapped, sizeof(swapped), hipMemcpyHostToDevice));
+        const double det_sw = DT::apply(d_m, nullptr, 2, linear_solver_type::LU_LINEAR_SOLVER, d_w, wsd);
+        HIP_OK(hipMemcpy(d_m, diag, sizeof(diag), hipMemcpyHostToDevice));
+        IV::apply(d_m, nullptr, 2, linear_solver_type::LU_LINEAR_SOLVER, d_w, wsd);
+        double inv[4] = {0, 0, 0, 0};
+        HIP_OK(hipMemcpy(inv, d_m, sizeof(inv), hipMemcpyDeviceToHost));
+        std::printf("determinant diag(4,2): %.17g, row-swapped: %.17g; inverse %g %g %g %g\n", det, det_sw, inv[0],
+            inv[1], inv[2], inv[3]);
+        EXPECT(det == 8.0);
+        EXPECT(det_sw == -8.0);
+        EXPECT(inv[0] == 0.25 && inv[1] == 0.0 && inv[2] == 0.0 && inv[3] == 0.5);
+        (void)hipFree(d_m);
         (void)hipFree(d_w);
     }
 }
