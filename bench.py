@@ -80,7 +80,7 @@ def parse():
     p.add_argument("--no-configs", action="store_true", help="skip the configuration grids")
     p.add_argument("--config-sizes", default="4096,8192,16384",
                    help="orders of the plain FP64 grid (nrhs 1 and 64, A iid N(0,1) and A + n I)")
-    p.add_argument("--consumer-sizes", default="1024,4096",
+    p.add_argument("--consumer-sizes", default="4096,16384",
                    help="orders of the determinant / inversion timing (comma-separated)")
     p.add_argument("--mixed-kappas", default="1e2,1e10",
                    help="further condition numbers of the mixed grid at the headline order")

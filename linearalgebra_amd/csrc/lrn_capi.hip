@@ -1182,9 +1182,50 @@ template <typename T> size_t inverse_ws_bytes(long long n, int type)
 {
     Arena ar;
     carve_consumer<T>(ar, n, type);
-    carve_getrs<T>(ar, n, n);
+    carve_getrs<T>(ar, n, 1);
     ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(n));
     return ar.off;
+}
+
+// LU X = X in place for all n columns of the n x n block X (row stride n), blocked like the
+// factorisation: 64-row strips solved by strip_trsm_kernel, the rest of each 256-row block
+// updated by a GEMM of depth 64, the rows beyond the block by one GEMM of depth 256.
+constexpr long long kInvNB = 256;
+
+template <typename T> int launch_strip(const T* LU, long long n, long long r0, int w, T* X, bool lower, hipStream_t s)
+{
+    const dim3 grid(static_cast<unsigned>((n + 63) / 64));
+    if (lower)
+        hipLaunchKernelGGL((strip_trsm_kernel<T, true>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
+    else
+        hipLaunchKernelGGL((strip_trsm_kernel<T, false>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T> int blocked_lu_solve_all(const T* LU, long long n, T* X, hipStream_t s)
+{
+    // forward: unit lower L, top to bottom
+    for (long long b0 = 0; b0 < n; b0 += kInvNB) {
+        const long long b1 = std::min(n, b0 + kInvNB);
+        for (long long s0 = b0; s0 < b1; s0 += kStripRows) {
+            const long long s1 = std::min(b1, s0 + kStripRows);
+            LRN_TRY(launch_strip<T>(LU, n, s0, static_cast<int>(s1 - s0), X, true, s));
+            LRN_TRY(launch_gemm_sub<T>(LU + s1 * n + s0, n, X + s0 * n, n, X + s1 * n, n, b1 - s1, n, s1 - s0, s));
+        }
+        LRN_TRY(launch_gemm_sub<T>(LU + b1 * n + b0, n, X + b0 * n, n, X + b1 * n, n, n - b1, n, b1 - b0, s));
+    }
+    // backward: non-unit upper U, bottom to top
+    for (long long b0 = (n - 1) / kInvNB * kInvNB; b0 >= 0; b0 -= kInvNB) {
+        const long long b1 = std::min(n, b0 + kInvNB);
+        for (long long s0 = b0 + (b1 - 1 - b0) / kStripRows * kStripRows; s0 >= b0; s0 -= kStripRows) {
+            const long long s1 = std::min(b1, s0 + kStripRows);
+            LRN_TRY(launch_strip<T>(LU, n, s0, static_cast<int>(s1 - s0), X, false, s));
+            LRN_TRY(launch_gemm_sub<T>(LU + b0 * n + s0, n, X + s0 * n, n, X + b0 * n, n, s0 - b0, n, s1 - s0, s));
+        }
+        LRN_TRY(launch_gemm_sub<T>(LU + b0, n, X + b0 * n, n, X, n, b0, n, b1 - b0, s));
+    }
+    return LRN_OK;
 }
 
 // Factor (LU mode) and return the pivots to use; info ends up in L.info on the device.
@@ -1242,8 +1283,8 @@ int determinant_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspac
     return LRN_OK;
 }
 
-// A^-1 in place: X = I, LU X = P X with the substitution kernels of the multi-RHS solve, then
-// X is copied over m unless the factorisation found a zero pivot (m then keeps the factors,
+// A^-1 in place: X = P I, L U X = X by the blocked strip solve and GEMM updates, then X is
+// copied over m unless the factorisation found a zero pivot (m then keeps the factors,
 // src/matrix_operation/cpu/native/inversion_native.cpp:421-436).
 template <typename T>
 int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace, size_t ws_size, void* stream)
@@ -1259,14 +1300,15 @@ int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace,
     const long long n = lda;
     Arena ar(workspace, ws_size);
     const auto L = carve_consumer<T>(ar, n, type);
-    const auto G = carve_getrs<T>(ar, n, n);
+    const auto G = carve_getrs<T>(ar, n, 1);
     T* X = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(n));
     const long long* piv = nullptr;
     LRN_TRY(consumer_factor<T>(m, pivot, n, type, L, s, &piv));
     const unsigned grid = static_cast<unsigned>(std::min<long long>((n * n + 255) / 256, 1ll << 16));
-    hipLaunchKernelGGL(eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n);
+    LRN_TRY(getrs_perm<T>(n, piv, G, s));
+    hipLaunchKernelGGL(perm_eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n, G.perm);
     LRN_HIP(hipGetLastError());
-    LRN_TRY(getrs_run<T>(m, n, piv, X, n, n, G, L.info, s));
+    LRN_TRY(blocked_lu_solve_all<T>(m, n, X, s));
     hipLaunchKernelGGL(copy_unless_kernel<T>, dim3(grid), dim3(256), 0, s, X, m, n * n, L.info);
     LRN_HIP(hipGetLastError());
     return consumer_status<T>(L, type, s);

@@ -45,6 +45,65 @@ __global__ __launch_bounds__(256) void lu_det_kernel(const T* __restrict__ LU, l
     }
 }
 
+// X = P I (n x n, row stride ldx): row i is the unit row e_perm[i], perm[i] being the row of
+// the input that lands in row i after the interchanges; grid-stride loop.
+template <typename T>
+__global__ __launch_bounds__(256) void perm_eye_kernel(T* __restrict__ X, long long n, long long ldx,
+    const int* __restrict__ perm)
+{
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; idx < n * n; idx += stride) {
+        const long long i = idx / n, j = idx - i * n;
+        X[i * ldx + j] = (static_cast<long long>(perm[i]) == j) ? T(1) : T(0);
+    }
+}
+
+constexpr int kStripRows = 64;
+
+// One row strip of a blocked triangular solve with many right-hand sides:
+// X[r0 : r0 + w, :] = D^-1 X[r0 : r0 + w, :] for the w x w diagonal block D = A[r0 : r0 + w,
+// r0 : r0 + w] (w <= 64), unit lower (kLower) or non-unit upper. One thread per column of X
+// (64-thread workgroups, consecutive columns so the row accesses coalesce), the column's w
+// values in registers, D staged in LDS and read as broadcasts.
+template <typename T, bool kLower>
+__global__ __launch_bounds__(64) void strip_trsm_kernel(const T* __restrict__ A, long long lda, long long r0, int w,
+    T* __restrict__ X, long long ldx, long long ncols)
+{
+    __shared__ T tri[kStripRows][kStripRows + 1];
+    for (int idx = threadIdx.x; idx < kStripRows * kStripRows; idx += 64) {
+        const int i = idx / kStripRows, j = idx - i * kStripRows;
+        tri[i][j] = (i < w && j < w) ? A[(r0 + i) * lda + r0 + j] : T(0);
+    }
+    __syncthreads();
+    const long long col = static_cast<long long>(blockIdx.x) * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    T x[kStripRows];
+#pragma unroll
+    for (int i = 0; i < kStripRows; ++i) x[i] = (i < w) ? X[(r0 + i) * ldx + col] : T(0);
+    if constexpr (kLower) {
+#pragma unroll
+        for (int i = 1; i < kStripRows; ++i) {
+            T acc = x[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) acc -= tri[i][k] * x[k];
+            x[i] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int i = kStripRows - 1; i >= 0; --i) {
+            if (i < w) {
+                T acc = x[i];
+#pragma unroll
+                for (int k = i + 1; k < kStripRows; ++k) acc -= tri[i][k] * x[k];
+                x[i] = acc / tri[i][i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kStripRows; ++i)
+        if (i < w) X[(r0 + i) * ldx + col] = x[i];
+}
+
 // X = I (n x n, row stride ldx); grid-stride loop.
 template <typename T>
 __global__ __launch_bounds__(256) void eye_kernel(T* __restrict__ X, long long n, long long ldx)
