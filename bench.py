@@ -81,7 +81,7 @@ def parse():
     p.add_argument("--config-sizes", default="4096,8192,16384",
                    help="orders of the plain FP64 grid (nrhs 1 and 64, A iid N(0,1) and A + n I)")
     p.add_argument("--consumer-sizes", default="4096,16384",
-                   help="orders of the determinant / inversion timing (comma-separated)")
+                   help="orders of the determinant / inversion / Cholesky timing (comma-separated)")
     p.add_argument("--mixed-kappas", default="1e2,1e10",
                    help="further condition numbers of the mixed grid at the headline order")
     p.add_argument("--mixed-extra-n", type=int, default=0,
@@ -375,6 +375,57 @@ def consumers_bench(torch, sizes, seed, dev, reps=2):
     return out
 
 
+def cholesky_bench(torch, sizes, seed, dev, reps=2):
+    """cholesky_decomposition<double, ROCM_NATIVE> (lower) and linear_solver in CHOLESKY mode
+    (factor + one right-hand side) on A = G G^T / n + I: median of `reps` HIP-event-timed calls
+    after a warmup, the FP64 rate of n^3/3 flops, and max|L L^T - A| / max|A|."""
+    import linearalgebra_amd as la
+    out = []
+    for n in sizes:
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed + 3 * n)
+        x = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        a = x @ x.T / n + torch.eye(n, dtype=torch.float64, device=dev)
+        del x
+        m = torch.empty_like(a)
+        ws = torch.empty(la.cholesky_decomposition.workspace_size(n), dtype=torch.uint8, device=dev)
+        b = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+        xs = torch.empty_like(b)
+        sws = torch.empty(la.linear_solver.workspace_size(n, la.linear_solver_type.CHOLESKY_LINEAR_SOLVER),
+                          dtype=torch.uint8, device=dev)
+        row = {"n": n}
+        for name in ("factor", "solve"):
+            times = []
+            for r in range(reps + 1):
+                m.copy_(a)
+                xs.copy_(b)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                if name == "factor":
+                    la.cholesky_decomposition.apply(m, n, workspace=ws)
+                else:
+                    la.linear_solver.apply(m, None, n, xs, la.linear_solver_type.CHOLESKY_LINEAR_SOLVER, sws)
+                e1.record()
+                torch.cuda.synchronize()
+                if r > 0:
+                    times.append(e0.elapsed_time(e1))
+            ms = sorted(times)[len(times) // 2]
+            row[f"{name}_ms"] = round(ms, 3)
+            if name == "factor":
+                row["factor_tflops"] = round(n ** 3 / 3.0 / (ms * 1e-3) / 1e12, 2)
+                row["residual"] = float((m @ m.T - a).abs().max() / a.abs().max())
+            else:
+                row["backward_error"] = float((a @ xs - b).abs().max() /
+                                              (a.abs().sum(1).max() * xs.abs().max() + b.abs().max()))
+        log(f"cholesky n={n}: factor {row['factor_ms']:.1f} ms ({row['factor_tflops']} TFLOP/s), "
+            f"factor+solve {row['solve_ms']:.1f} ms, residual {row['residual']:.1e}, berr {row['backward_error']:.1e}")
+        out.append(row)
+        del a, m, ws, b, xs, sws
+    return out
+
+
 # ----------------------------------------------------------------------------- batched
 
 def batched_bench(torch, count, replays, seed, dev):
@@ -632,6 +683,7 @@ def main():
     mixed_configs = None
     triangular = None
     consumers = None
+    cholesky = None
     if not args.no_configs:
         plain_configs = plain_configs_bench(torch, [int(v) for v in args.config_sizes.split(",") if v],
                                             (1, 64), args.seed, dev)
@@ -639,6 +691,7 @@ def main():
         mixed_configs = mixed_configs_bench(torch, [n], kappas, args.seed, dev, args.restart) if kappas else []
         triangular = triangular_bench(torch, n, args.seed, dev)
         consumers = consumers_bench(torch, [int(v) for v in args.consumer_sizes.split(",") if v], args.seed, dev)
+        cholesky = cholesky_bench(torch, [int(v) for v in args.consumer_sizes.split(",") if v], args.seed, dev)
         if args.mixed_extra_n > 0:
             mixed_configs += mixed_configs_bench(torch, [args.mixed_extra_n], [1e2, args.kappa] + [k for k in kappas if k > 1e6],
                                                  args.seed, dev, args.restart)
@@ -696,6 +749,7 @@ def main():
         "mixed_configs": mixed_configs,
         "triangular_solver": triangular,
         "determinant_inversion": consumers,
+        "cholesky": cholesky,
         "batched": batched,
         "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,

@@ -73,7 +73,10 @@ LRN_API int lrn_lu_decomposition_f32(float* m, int64_t lda, int64_t* pivot, void
  * side `x` of lda values, overwritten with the solution. LU mode factors `m` in place
  * (pivots to `pivot`) and solves only if the matrix is nonsingular, leaving x unchanged
  * otherwise (linear_solver_native.cpp:160-171); LU_UPFRONT reuses the caller's factors
- * and pivots. ASYNCHRONOUS on `stream`. Cholesky modes return LRN_E_UNSUPPORTED. */
+ * and pivots. ASYNCHRONOUS on `stream`. Cholesky modes (linear_solver_native.cpp:177-189): the
+ * lower factor (CHOLESKY factors m in place first, as lrn_cholesky_decomposition with
+ * LOWER, and solves only on success; CHOLESKY_UPFRONT reads the caller's lower factor), then
+ * L y = b and L^T x = y; `pivot` is not used and may be NULL. */
 LRN_API int lrn_linear_solver_workspace_size_f64(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_linear_solver_workspace_size_f32(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double* x, int type,
@@ -98,6 +101,22 @@ LRN_API int lrn_triangular_solver_f64(int upper, int transpose, int unit, int64_
 LRN_API int lrn_triangular_solver_f32(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const float* a,
     int64_t lda, const float* b, int64_t ldb, float* x, int64_t ldx, void* workspace, size_t workspace_size,
     void* stream);
+
+/* ---- cholesky_decomposition<T, ROCM_NATIVE> ---------------------------------------------
+ * Replaces cholesky_decomposition<T, B>::apply / workspace_size
+ * (include/matrix_operation/cholesky_decomposition.h; algorithm src/matrix_operation/cpu/
+ * native/cholesky_decomposition_native.cpp:245-320). C row-major lda x lda, only its lower
+ * triangle read; upper = 0 writes L (A = L L^T) with zeros above the diagonal, upper = 1 writes
+ * U = L^T (A = U^T U) with zeros below. *info (host, may be NULL) = 0 on success, else the
+ * 1-based order of the first leading minor whose pivot is not positive (the factor is then
+ * incomplete and C unspecified). Argument errors: -1 C, -2 lda, -3 upper, -4 workspace.
+ * SYNCHRONOUS on stream (info is returned on the host, as lu_decomposition does). */
+LRN_API int lrn_cholesky_decomposition_workspace_size_f64(int64_t lda, size_t* bytes);
+LRN_API int lrn_cholesky_decomposition_workspace_size_f32(int64_t lda, size_t* bytes);
+LRN_API int lrn_cholesky_decomposition_f64(double* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
+    int* info, void* stream);
+LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
+    int* info, void* stream);
 
 /* ---- determinant<T, ROCM_NATIVE> -------------------------------------------------------
  * Replaces determinant<T, B>::apply / workspace_size (include/matrix_operation/determinant.h:

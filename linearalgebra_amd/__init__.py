@@ -6,6 +6,8 @@ loads it and passes device pointers and streams. There is no CPU fallback.
 """
 from .lu import (  # noqa: F401
     LinalgError,
+    cholesky_decomposition,
+    cholesky_decomposition_enum,
     get_tuning,
     linear_solver,
     linear_solver_type,
@@ -17,5 +19,5 @@ from .mixed import gesv_mixed  # noqa: F401,E402
 from .triangular import triangular_solver  # noqa: F401,E402
 from .inverse import determinant, inversion  # noqa: F401,E402
 
-__all__ = ["determinant", "inversion", "gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type",
+__all__ = ["cholesky_decomposition", "cholesky_decomposition_enum", "determinant", "inversion", "gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type",
            "triangular_solver", "LinalgError", "set_tuning", "get_tuning"]

@@ -38,6 +38,7 @@
 #include "lrn_batched.hpp"
 #include "lrn_calu.hpp"
 #include "lrn_inverse.hpp"
+#include "lrn_cholesky.hpp"
 
 namespace lrn {
 namespace {
@@ -856,13 +857,7 @@ template <typename T> size_t getrs_ws_bytes(long long n, long long nrhs)
     return ar.off;
 }
 
-template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int type)
-{
-    Arena ar;
-    if (type == LRN_LU_LINEAR_SOLVER) carve_getrf<T>(ar);
-    carve_getrs<T>(ar, n, nrhs);
-    return ar.off;
-}
+template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int type);
 
 // Single-RHS substitution with FP32 factors in FP64 arithmetic (the refinement's
 // preconditioner): 0 = the 64-row kernel (the default), 1 = 256-row chained kernel (streams
@@ -985,10 +980,10 @@ size_t trsm_ws_bytes(long long n)
 
 template <typename T, bool kFwd, bool kUnit, bool kTrans>
 int launch_trsm_chain(const T* a, long long lda, long long n, const T* b, long long ldb, T* x, long long ldx, int nc,
-    unsigned* flags, int* err, hipStream_t s)
+    unsigned* flags, int* err, hipStream_t s, const int* skip = nullptr)
 {
     hipLaunchKernelGGL((trsv_blocked_kernel<T, T, kFwd, kUnit, kTrans>), dim3(static_cast<unsigned>(solve_blocks(n))),
-        dim3(256), 0, s, a, lda, n, nullptr, b, ldb, x, ldx, nc, flags, kSpinLimit, err, nullptr);
+        dim3(256), 0, s, a, lda, n, nullptr, b, ldb, x, ldx, nc, flags, kSpinLimit, err, skip);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -1037,6 +1032,105 @@ int triangular_solver_impl(int upper, int transpose, int unit, long long n, long
     return LRN_OK;
 }
 
+
+// ------------------------------------------------------------- Cholesky
+// A = L L^T, row-major, lower triangle read (reference: include/matrix_operation/
+// cholesky_decomposition.h; algorithm src/matrix_operation/cpu/native/
+// cholesky_decomposition_native.cpp:245-320): 64-wide block columns, each the diagonal block
+// (chol_diag_kernel), the rows below it (chol_rows_kernel, also written transposed into Wt)
+// and the trailing update A22 -= L21 L21^T on the GEMM kernels (the whole square: the upper
+// part it writes is never read and is overwritten by the final layout pass).
+long long chol_ldw(long long n) { return (n + 3) / 4 * 4; }
+
+template <typename T> struct CholLayout {
+    int* info = nullptr;
+    T* wt = nullptr;
+};
+
+template <typename T> CholLayout<T> carve_chol(Arena& ar, long long n)
+{
+    CholLayout<T> L;
+    L.info = ar.next<int>(64);
+    L.wt = ar.next<T>(static_cast<size_t>(kCholNB) * static_cast<size_t>(chol_ldw(n)));
+    return L;
+}
+
+template <typename T> size_t chol_ws_bytes(long long n)
+{
+    Arena ar;
+    carve_chol<T>(ar, n);
+    return ar.off;
+}
+
+// Factor into the lower triangle (info on the device, zero on success), then write the
+// layout (upper = 1: U = L^T in the upper triangle).
+template <typename T> int chol_factor(T* A, long long n, int upper, const CholLayout<T>& L, hipStream_t s)
+{
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.info), 0u, 64, s));
+    const long long ldw = chol_ldw(n);
+    for (long long k = 0; k < n; k += kCholNB) {
+        const int bs = static_cast<int>(std::min<long long>(kCholNB, n - k));
+        hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(256), 0, s, A, n, k, bs, L.info);
+        LRN_HIP(hipGetLastError());
+        const long long m = n - k - bs;
+        if (m <= 0) break;
+        hipLaunchKernelGGL(chol_rows_kernel<T>, dim3(static_cast<unsigned>((m + 63) / 64)), dim3(64), 0, s, A, n, k,
+            bs, n, L.wt, ldw, L.info);
+        LRN_HIP(hipGetLastError());
+        LRN_TRY(launch_gemm_sub<T>(A + (k + bs) * n + k, n, L.wt, ldw, A + (k + bs) * n + (k + bs), n, m, m, bs, s));
+    }
+    const long long pairs = n * (n - 1) / 2;
+    const unsigned grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>((pairs + 255) / 256, 1ll << 16)));
+    hipLaunchKernelGGL(chol_layout_kernel<T>, dim3(grid), dim3(256), 0, s, A, n, upper, L.info);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int type)
+{
+    Arena ar;
+    if (type == LRN_LU_LINEAR_SOLVER) carve_getrf<T>(ar);
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER) carve_chol<T>(ar, n);
+    carve_getrs<T>(ar, n, nrhs);
+    return ar.off;
+}
+
+template <typename T>
+int cholesky_impl(T* c, int64_t lda, int upper, void* workspace, size_t ws_size, int* info, void* stream)
+{
+    if (c == nullptr) return -1;
+    if (lda <= 0) return -2;
+    if (upper != 0 && upper != 1) return -3;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < chol_ws_bytes<T>(lda)) return -4;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const auto L = carve_chol<T>(ar, lda);
+    LRN_TRY(chol_factor<T>(c, lda, upper, L, s));
+    int host = 0;
+    LRN_HIP(hipMemcpyAsync(&host, L.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipStreamSynchronize(s));
+    if (info != nullptr) *info = host;
+    return LRN_OK;
+}
+
+// L L^T X = B in place (B row-major n x nrhs, stride ldb) with the lower factor in m: forward
+// with L, backward with L^T (transposed reads of the same triangle), 64 columns per launch
+// through G.y; skipped while *skip is nonzero.
+template <typename T>
+int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& G,
+    const int* skip, hipStream_t s)
+{
+    for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
+        const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
+        LRN_TRY(launch_fill(G.flags_f, 0u, solve_blocks(n), s));
+        LRN_TRY(launch_fill(G.flags_b, 0u, solve_blocks(n), s));
+        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
+        LRN_TRY((launch_trsm_chain<T, true, false, false>(m, n, n, B + c0, ldb, G.y, nc, nc, G.flags_f, G.err, s, skip)));
+        LRN_TRY((launch_trsm_chain<T, false, false, true>(m, n, n, G.y, nc, B + c0, ldb, nc, G.flags_b, G.err, s, skip)));
+    }
+    return LRN_OK;
+}
 
 template <typename T>
 int lu_decomposition_impl(T* m, int64_t lda, int64_t* pivot, void* workspace, size_t ws_size, int* info, void* stream)
@@ -1126,14 +1220,25 @@ int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t l
     if (b == nullptr) return -4;
     if (nrhs < 0) return -5;
     if (ldb < std::max<int64_t>(1, nrhs)) return -6;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (type != LRN_LU_LINEAR_SOLVER && type != LRN_LU_UPFRONT_LINEAR_SOLVER) return -7;
-    if (pivot == nullptr) return -2;
+    const bool chol = type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER;
+    if (!chol && type != LRN_LU_LINEAR_SOLVER && type != LRN_LU_UPFRONT_LINEAR_SOLVER) return -7;
+    if (pivot == nullptr && !chol) return -2;
     if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
     if (workspace == nullptr || ws_size < solver_ws_bytes<T>(lda, std::max<int64_t>(nrhs, 1), type)) return -8;
     if (nrhs == 0) return LRN_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     Arena ar(workspace, ws_size);
+    if (chol) {
+        // linear_solver_native.cpp:177-189: factor the lower triangle, solve only on success
+        const int* skip = nullptr;
+        if (type == LRN_CHOLESKY_LINEAR_SOLVER) {
+            const auto C = carve_chol<T>(ar, lda);
+            LRN_TRY(chol_factor<T>(m, lda, 0, C, s));
+            skip = C.info;
+        }
+        const auto G = carve_getrs<T>(ar, lda, nrhs);
+        return chol_solve<T>(m, lda, b, nrhs, ldb, G, skip, s);
+    }
     const int* skip = nullptr;
     if (type == LRN_LU_LINEAR_SOLVER) {
         const auto F = carve_getrf<T>(ar);
@@ -1872,6 +1977,31 @@ LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, f
     int type, void* workspace, size_t workspace_size, void* stream)
 {
     return solver_impl<float>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_cholesky_decomposition_workspace_size_f64(int64_t lda, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = lrn::chol_ws_bytes<double>(lda);
+    return LRN_OK;
+}
+LRN_API int lrn_cholesky_decomposition_workspace_size_f32(int64_t lda, size_t* bytes)
+{
+    if (lda <= 0) return -1;
+    if (bytes == nullptr) return -2;
+    *bytes = lrn::chol_ws_bytes<float>(lda);
+    return LRN_OK;
+}
+LRN_API int lrn_cholesky_decomposition_f64(double* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
+    int* info, void* stream)
+{
+    return lrn::cholesky_impl<double>(c, lda, upper, workspace, workspace_size, info, stream);
+}
+LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
+    int* info, void* stream)
+{
+    return lrn::cholesky_impl<float>(c, lda, upper, workspace, workspace_size, info, stream);
 }
 
 LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes)

@@ -1,0 +1,129 @@
+// Kernels of the blocked Cholesky factorisation A = L L^T (row-major, lower triangle read):
+// per 64-wide block column, the diagonal block in one workgroup, the rows below it solved
+// against that block (also written transposed for the trailing GEMM), then the trailing
+// update on the GEMM kernels of the LU path. A final pass writes the triangle layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lrn {
+
+constexpr int kCholNB = 64;
+
+// Diagonal block A[k : k + bs, k : k + bs] (bs <= 64) factored in LDS, right-looking: at step
+// j the updated diagonal d = a_jj - sum_{p<j} l_jp^2 fails the factorisation when d <= 0
+// (the reference's test, src/matrix_operation/cpu/native/cholesky_decomposition_native.cpp:
+// 139-142; NaN passes it), *info = k + j + 1; otherwise l_jj = sqrt(d), the column below is
+// divided by it and the rest of the block's lower triangle takes the rank-1 update. Only the
+// lower triangle is read or written. Skipped when *info is already nonzero.
+template <typename T>
+__global__ __launch_bounds__(256) void chol_diag_kernel(T* __restrict__ A, long long lda, long long k, int bs,
+    int* __restrict__ info)
+{
+    __shared__ T blk[kCholNB][kCholNB + 1];
+    __shared__ int fail;
+    if (*info != 0) return;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < kCholNB * kCholNB; idx += 256) {
+        const int i = idx / kCholNB, j = idx - i * kCholNB;
+        blk[i][j] = (i < bs && j <= i) ? A[(k + i) * lda + k + j] : T(0);
+    }
+    if (t == 0) fail = 0;
+    __syncthreads();
+    for (int j = 0; j < bs; ++j) {
+        const T d = blk[j][j];
+        if (!(d <= T(0))) {
+            const T l = sqrt(d);
+            __syncthreads();
+            if (t == 0) blk[j][j] = l;
+            for (int i = j + 1 + t; i < bs; i += 256) blk[i][j] = blk[i][j] / l;
+            __syncthreads();
+            // lower triangle of the remaining block: (i, c) with j < c <= i < bs
+            const int rem = bs - j - 1;
+            for (int idx = t; idx < rem * rem; idx += 256) {
+                const int i = j + 1 + idx / rem, c = j + 1 + idx % rem;
+                if (c <= i) blk[i][c] = blk[i][c] - blk[i][j] * blk[c][j];
+            }
+            __syncthreads();
+        } else {
+            if (t == 0) {
+                fail = j + 1;
+                info[0] = static_cast<int>(k + j + 1);
+            }
+            __syncthreads();
+            break;
+        }
+    }
+    if (fail != 0) return;
+    for (int idx = t; idx < kCholNB * kCholNB; idx += 256) {
+        const int i = idx / kCholNB, j = idx - i * kCholNB;
+        if (i < bs && j <= i) A[(k + i) * lda + k + j] = blk[i][j];
+    }
+}
+
+// Rows r in [k + bs, n) of the block column: x L11^T = a, i.e. x_j = (a_j - sum_{p<j} x_p l_jp)
+// / l_jj in increasing j (cholesky_decomposition_native.cpp:278-286), written back in place and
+// transposed into Wt (bs x m, row stride ldw, column r - k - bs) for the trailing GEMM. 64
+// rows per workgroup, staged through LDS so the row reads and writes coalesce; L11 read as
+// broadcasts. Skipped when *info is nonzero.
+template <typename T>
+__global__ __launch_bounds__(64) void chol_rows_kernel(T* __restrict__ A, long long lda, long long k, int bs,
+    long long n, T* __restrict__ Wt, long long ldw, const int* __restrict__ info)
+{
+    __shared__ T l11[kCholNB][kCholNB + 1];
+    __shared__ T tile[kCholNB][kCholNB + 1];
+    if (*info != 0) return;
+    const int t = threadIdx.x;
+    const long long r0 = k + bs + static_cast<long long>(blockIdx.x) * 64;
+    const int rows = static_cast<int>(n - r0 < 64 ? n - r0 : 64);
+    for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
+        const int i = idx / kCholNB, j = idx - i * kCholNB;
+        l11[i][j] = (i < bs && j <= i) ? A[(k + i) * lda + k + j] : T(0);
+        tile[i][j] = (i < rows && j < bs) ? A[(r0 + i) * lda + k + j] : T(0);
+    }
+    __syncthreads();
+    if (t < rows) {
+        for (int j = 0; j < bs; ++j) {
+            T acc = tile[t][j];
+            for (int p = 0; p < j; ++p) acc = acc - tile[t][p] * l11[j][p];
+            tile[t][j] = acc / l11[j][j];
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
+        const int i = idx / kCholNB, j = idx - i * kCholNB;
+        if (i < rows && j < bs) A[(r0 + i) * lda + k + j] = tile[i][j];
+    }
+    // transposed copy: row j of Wt holds column j of the solved rows
+    for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
+        const int j = idx / kCholNB, i = idx - j * kCholNB;
+        if (i < rows && j < bs) Wt[j * ldw + (r0 - k - bs) + i] = tile[i][j];
+    }
+}
+
+// Final layout: lower = zero the strict upper triangle; upper = U[j][i] = L[i][j] and the
+// strict lower triangle zeroed (cholesky_decomposition_native.cpp:262-265,312-319). One thread
+// per pair (i > j), grid-stride. Skipped when *info is nonzero (the factors are incomplete).
+template <typename T>
+__global__ __launch_bounds__(256) void chol_layout_kernel(T* __restrict__ A, long long n, int upper,
+    const int* __restrict__ info)
+{
+    if (*info != 0) return;
+    const long long pairs = n * (n - 1) / 2;
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; q < pairs; q += stride) {
+        // q -> (i, j), i > j, rows in order: row i holds pairs [i(i-1)/2, i(i+1)/2)
+        long long i = static_cast<long long>((1.0 + sqrt(1.0 + 8.0 * static_cast<double>(q))) * 0.5);
+        while (i * (i - 1) / 2 > q) --i;
+        while ((i + 1) * i / 2 <= q) ++i;
+        const long long j = q - i * (i - 1) / 2;
+        if (upper) {
+            A[j * n + i] = A[i * n + j];
+            A[i * n + j] = T(0);
+        } else {
+            A[j * n + i] = T(0);
+        }
+    }
+}
+
+}  // namespace lrn

@@ -111,7 +111,8 @@ class lu_decomposition:
 
 
 class linear_solver:
-    """linear_solver<T, ROCM_NATIVE>: LU modes; x (or an lda x nrhs block) is overwritten."""
+    """linear_solver<T, ROCM_NATIVE>: LU and Cholesky modes; x (or an lda x nrhs block) is overwritten.
+    Cholesky modes use the lower factor and ignore ``pivot`` (may be None)."""
 
     @staticmethod
     def workspace_size(lda: int, type: linear_solver_type, dtype: torch.dtype = torch.float64, nrhs: int = 1) -> int:
@@ -137,7 +138,8 @@ class linear_solver:
         if x.dtype != m.dtype or not x.is_cuda or not x.is_contiguous() or x.shape[0] < lda:
             raise ValueError("linear_solver::apply: x must be a contiguous device array of lda rows, same dtype as m")
         type = linear_solver_type(int(type))
-        if pivot is None or not pivot.is_cuda or pivot.dtype != torch.int64:
+        chol = type in (linear_solver_type.CHOLESKY_LINEAR_SOLVER, linear_solver_type.CHOLESKY_UPFRONT_LINEAR_SOLVER)
+        if not (chol and pivot is None) and (pivot is None or not pivot.is_cuda or pivot.dtype != torch.int64):
             raise ValueError("linear_solver::apply: pivot must be a device int64 array")
         nrhs = 1 if x.dim() == 1 else int(x.shape[1])
         if workspace is None:
@@ -153,6 +155,47 @@ class linear_solver:
                 _ptr(m), _ptr(pivot), int(lda), _ptr(x), nrhs, nrhs, int(type), _ptr(workspace), ws_bytes,
                 _stream_ptr(stream))
         _check(code, "linear_solver::apply")
+
+
+class cholesky_decomposition_enum(enum.Enum):
+    LOWER_TRIANGULAR = "L"
+    UPPER_TRIANGULAR = "U"
+
+
+class cholesky_decomposition:
+    """cholesky_decomposition<T, ROCM_NATIVE> (include/matrix_operation/cholesky_decomposition.h):
+    the selected factor in place (L with A = L L^T, or U with A = U^T U), zeros in the other
+    triangle; only the lower triangle of the input is read. Synchronous; returns True on
+    success and appends the LAPACK-style info (first non-positive-definite leading minor,
+    1-based, 0 on success) to ``info`` when a list is given."""
+
+    @staticmethod
+    def workspace_size(lda: int, dtype: torch.dtype = torch.float64) -> int:
+        out = ctypes.c_size_t(0)
+        fn = getattr(_lib.load(), f"lrn_cholesky_decomposition_workspace_size_{_sfx(dtype)}")
+        _check(fn(int(lda), ctypes.byref(out)), "cholesky_decomposition::workspace_size")
+        return int(out.value)
+
+    @staticmethod
+    def apply(c: torch.Tensor, lda: int, type: cholesky_decomposition_enum = cholesky_decomposition_enum.LOWER_TRIANGULAR,
+              workspace: torch.Tensor | None = None, workspace_size: int | None = None, info: list | None = None,
+              stream=None) -> bool:
+        _check_matrix(c, lda, "cholesky_decomposition::apply")
+        if not isinstance(type, cholesky_decomposition_enum):
+            raise ValueError("cholesky_decomposition::apply: Unsupported enum type!")
+        sfx = _sfx(c.dtype)
+        if workspace is None:
+            workspace = torch.empty(cholesky_decomposition.workspace_size(lda, c.dtype), dtype=torch.uint8,
+                                    device=c.device)
+        ws_bytes = workspace.numel() * workspace.element_size() if workspace_size is None else int(workspace_size)
+        inf = ctypes.c_int(0)
+        upper = 1 if type == cholesky_decomposition_enum.UPPER_TRIANGULAR else 0
+        code = getattr(_lib.load(), f"lrn_cholesky_decomposition_{sfx}")(
+            _ptr(c), int(lda), upper, _ptr(workspace), ws_bytes, ctypes.byref(inf), _stream_ptr(stream))
+        _check(code, "cholesky_decomposition::apply")
+        if info is not None:
+            info.append(int(inf.value))
+        return inf.value == 0
 
 
 def set_tuning(outer_nb: int = 0, panel_w: int = 0, panel_max_wg: int = 0) -> None:

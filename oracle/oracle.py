@@ -17,6 +17,10 @@ Reference correspondence:
              Testing/Cxx/TestGpuMatrixOperations.cxx:1466-1492) and numpy.linalg.det
   inverse_from_factors -> lu_invert (src/matrix_operation/cpu/native/inversion_native.cpp:229-304);
              pinned by A A^-1 = I on the reference's inversion input and numpy.linalg.inv
+  cholesky, cholesky_solve -> cholesky_decomposition<T, NATIVE> / cholesky_solve
+             (cholesky_decomposition_native.cpp:245-320, linear_solver_native.cpp:116-147);
+             pinned by the reference's known answer (the factor of C C^T is C,
+             Testing/Cxx/TestCholeskyDecomposition.cxx:170-210) and numpy.linalg.cholesky
 """
 from __future__ import annotations
 
@@ -404,3 +408,63 @@ def inversion_spd_case(dim: int, dtype=np.float64, seed: int = 0) -> np.ndarray:
             r[i, j] = 0.4 * rng.random()
     r = r.astype(dtype)
     return (r @ r.T).astype(dtype)
+
+
+# ------------------------------------------------------------------- Cholesky
+
+_CHOL_BLOCK = 64  # cholesky_decomposition_native.cpp:165
+
+
+def cholesky(a: np.ndarray, upper: bool = False):
+    """A = L L^T the way cholesky_decomposition_blocked_lower does it
+    (src/matrix_operation/cpu/native/cholesky_decomposition_native.cpp:245-320): 64-wide
+    blocks; in a block, row i takes l_ij = (a_ij - l_i. . l_j.) / l_jj for j < i and
+    l_ii = sqrt(a_ii - l_i. . l_i.), failing at the first row whose sum is <= 0; the rows below
+    solve against the block the same way; the trailing lower triangle takes a22 -= l21 l21^T.
+    Only the lower triangle of ``a`` is read. Returns (factor, info): L with zeros above the
+    diagonal (or U = L^T with zeros below when upper), info the 1-based failing order or 0."""
+    n = a.shape[0]
+    T = a.dtype.type
+    m = np.tril(np.array(a, dtype=a.dtype, copy=True))
+    for k in range(0, n, _CHOL_BLOCK):
+        bs = min(_CHOL_BLOCK, n - k)
+        for i in range(k, k + bs):
+            for j in range(k, i):
+                m[i, j] = T((m[i, j] - T(np.dot(m[i, k:j], m[j, k:j]))) / m[j, j])
+            s = T(m[i, i] - T(np.dot(m[i, k:i], m[i, k:i])))
+            if s <= 0:
+                return m, k + (i - k) + 1
+            m[i, i] = T(np.sqrt(s))
+        for r in range(k + bs, n):
+            for j in range(k, k + bs):
+                m[r, j] = T((m[r, j] - T(np.dot(m[r, k:j], m[j, k:j]))) / m[j, j])
+        if k + bs < n:
+            l21 = m[k + bs:, k:k + bs]
+            m[k + bs:, k + bs:] -= np.tril(l21 @ l21.T).astype(a.dtype)
+    return (np.ascontiguousarray(m.T) if upper else m), 0
+
+
+def cholesky_solve(l: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """L L^T x = b with the lower factor (linear_solver_native.cpp:116-147): forward with the
+    rows of L, then backward with its columns; b is (n,) or (n, nrhs)."""
+    n = l.shape[0]
+    x = np.array(b, dtype=l.dtype, copy=True)
+    for i in range(n):
+        x[i] = (x[i] - l[i, :i] @ x[:i]) / l[i, i]
+    for i in range(n - 1, -1, -1):
+        x[i] = (x[i] - l[i + 1:, i] @ x[i + 1:]) / l[i, i]
+    return x
+
+
+def cholesky_known_answer(dim: int, dtype=np.float64, seed: int = 0):
+    """The reference suite's Cholesky case (Testing/Cxx/TestCholeskyDecomposition.cxx:170-210,
+    315-330): C lower triangular with 5 on the diagonal and 0.5 u below (u uniform in [0, 1],
+    seeded numpy draws standing in for rand()), A = C C^T in the working type; the factor of A
+    must be C to 200 eps (absolute, per entry). Returns (a, c)."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((dim, dim), dtype=dtype)
+    for i in range(dim):
+        c[i, i] = 5.0
+        for j in range(i):
+            c[i, j] = 0.5 * rng.random()
+    return (c @ c.T).astype(dtype), c

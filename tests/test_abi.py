@@ -88,8 +88,20 @@ def test_argument_errors_before_device_work():
     assert lib.lrn_lu_decomposition_f64(dummy, 0, dummy, dummy, 1 << 20, ctypes.byref(info), nil) == -2
     assert lib.lrn_lu_decomposition_f64(dummy, 4, nil, dummy, 1 << 20, ctypes.byref(info), nil) == -3
     assert lib.lrn_lu_decomposition_f64(dummy, 4, dummy, dummy, 1, ctypes.byref(info), nil) == -4
-    # Cholesky modes are not part of this backend
-    assert lib.lrn_linear_solver_f64(dummy, dummy, 4, dummy, 3, dummy, 1 << 20, nil) == 3
+    # Cholesky modes take no pivot array; a too-small workspace is still caught first
+    assert lib.lrn_linear_solver_f64(dummy, nil, 4, dummy, 3, dummy, 1, nil) == -8
+    assert lib.lrn_linear_solver_f64(dummy, nil, 4, dummy, 1, dummy, 1 << 20, nil) == -2
+    # Cholesky factorisation: null C, bad lda, bad triangle, too-small workspace
+    assert lib.lrn_cholesky_decomposition_f64(nil, 4, 0, dummy, 1 << 20, ctypes.byref(info), nil) == -1
+    assert lib.lrn_cholesky_decomposition_f64(dummy, 0, 0, dummy, 1 << 20, ctypes.byref(info), nil) == -2
+    assert lib.lrn_cholesky_decomposition_f64(dummy, 4, 2, dummy, 1 << 20, ctypes.byref(info), nil) == -3
+    assert lib.lrn_cholesky_decomposition_f64(dummy, 4, 0, dummy, 1, ctypes.byref(info), nil) == -4
+    # determinant / inversion: Cholesky modes not provided, unknown type, missing UPFRONT pivots
+    det = ctypes.c_double(0.0)
+    assert lib.lrn_determinant_f64(dummy, nil, 4, 3, dummy, 1 << 20, ctypes.byref(det), nil) == 3
+    assert lib.lrn_determinant_f64(dummy, nil, 4, 7, dummy, 1 << 20, ctypes.byref(det), nil) == -4
+    assert lib.lrn_determinant_f64(dummy, nil, 4, 2, dummy, 1 << 20, ctypes.byref(det), nil) == -2
+    assert lib.lrn_inversion_f64(dummy, nil, 4, 1, dummy, 1, nil) == -5
     # unknown solver type
     assert lib.lrn_linear_solver_f64(dummy, dummy, 4, dummy, 9, dummy, 1 << 20, nil) == -7
     # mixed solve argument checks

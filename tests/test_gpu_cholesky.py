@@ -1,0 +1,138 @@
+"""cholesky_decomposition<T, ROCM_NATIVE> and the Cholesky modes of linear_solver on the GPU
+against the reference's known answer and the oracle.
+
+Bar: the reference's (TestCholeskyDecomposition.cxx:170-210): the factor of C C^T is C within
+200 eps per entry (order 198, FP64 and FP32, both triangles); against the oracle 1e-13
+normwise (FP64); the opposite triangle zero; only the input's lower triangle read; info the
+first failing leading minor; solves within 1e-12 of the oracle's and the reference's linear
+solver bar (TestLinearSolver.cxx:122-135, x error <= 1e-10 at order 198).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _chol_gpu(a, dev, upper=False):
+    import torch
+    import linearalgebra_amd as la
+    n = a.shape[0]
+    m = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    info = []
+    kind = la.cholesky_decomposition_enum.UPPER_TRIANGULAR if upper else la.cholesky_decomposition_enum.LOWER_TRIANGULAR
+    ok = la.cholesky_decomposition.apply(m, n, kind, info=info)
+    return m.cpu().numpy(), info[0], ok
+
+
+def _solve_gpu(m_in, b, dev, type):
+    import torch
+    import linearalgebra_amd as la
+    n = m_in.shape[0]
+    m = torch.from_numpy(np.ascontiguousarray(m_in)).to(dev)
+    x = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
+    la.linear_solver.apply(m, None, n, x, la.linear_solver_type(type))
+    torch.cuda.synchronize()
+    return m.cpu().numpy(), x.cpu().numpy()
+
+
+@pytest.mark.parametrize("dim", [1, 13, 65, 198, 300])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("upper", [False, True])
+def test_cholesky_reference_known_answer(dim, dtype, upper, dev):
+    a, c = O.cholesky_known_answer(dim, dtype, seed=dim)
+    r, info, ok = _chol_gpu(a, dev, upper)
+    assert ok and info == 0
+    want = c.T if upper else c
+    err = np.abs(r.astype(np.float64) - want).max()
+    print(f"cholesky dim={dim} {np.dtype(dtype).name} upper={upper}: max|R - C| {err:.2e}", flush=True)
+    assert err <= 200 * np.finfo(dtype).eps
+    other = np.tril_indices(dim, -1) if upper else np.triu_indices(dim, 1)
+    assert (r[other] == 0).all()
+
+
+@pytest.mark.parametrize("n", [64, 130, 300, 1000])
+def test_cholesky_matches_oracle_and_reads_lower_only(n, dev):
+    rng = np.random.default_rng(n)
+    g = rng.standard_normal((n, n))
+    a = g @ g.T / n + np.eye(n)
+    poisoned = a.copy()
+    poisoned[np.triu_indices(n, 1)] = np.nan
+    r, info, ok = _chol_gpu(poisoned, dev)
+    assert ok and info == 0
+    ref, _ = O.cholesky(a) if n <= 300 else (np.linalg.cholesky(a), 0)
+    d = O.rel_diff(r, ref)
+    print(f"cholesky n={n}: rel diff {d:.2e}", flush=True)
+    assert d <= 1e-13
+    assert (r[np.triu_indices(n, 1)] == 0).all()
+
+
+def test_cholesky_info(dev):
+    a = np.eye(5)
+    a[3, 3] = -1.0
+    _, info, ok = _chol_gpu(a, dev)
+    assert info == 4 and not ok
+    b = np.eye(70)
+    b[66, 66] = 0.0
+    assert _chol_gpu(b, dev)[1] == 67
+    c = np.eye(300)
+    c[299, 299] = -5.0
+    assert _chol_gpu(c, dev)[1] == 300
+
+
+def test_cholesky_large_residual(dev):
+    import torch
+    import linearalgebra_amd as la
+    n = 3000
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    x = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    a = x @ x.T / n + torch.eye(n, dtype=torch.float64, device=dev)
+    m = a.clone()
+    assert la.cholesky_decomposition.apply(m, n)
+    res = float((m @ m.T - a).abs().max() / a.abs().max())
+    print(f"cholesky n={n}: max|L L^T - A| / max|A| {res:.2e}", flush=True)
+    assert res <= 1e-14
+    assert float(m.triu(1).abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 70])
+def test_cholesky_solver_modes(nrhs, dev):
+    n = 198
+    a = O.inversion_spd_case(n, seed=7)  # TestLinearSolver.cxx's input: R R^T, unit-lower R
+    x0 = np.sin(np.arange(n * nrhs).reshape(n, nrhs) + 1.0)
+    b = a @ x0
+    bb = b[:, 0] if nrhs == 1 else b
+    lo, x = _solve_gpu(a, bb, dev, 3)  # CHOLESKY_LINEAR_SOLVER
+    ref_l, _ = O.cholesky(a)
+    assert O.rel_diff(lo, ref_l) <= 1e-13
+    x = x.reshape(n, nrhs)
+    assert np.abs(x - x0).max() <= 1e-10
+    assert O.rel_diff(x, O.cholesky_solve(ref_l, b)) <= 1e-12
+    # CHOLESKY_UPFRONT with the caller's lower factor
+    _, x2 = _solve_gpu(ref_l, bb, dev, 4)
+    assert O.rel_diff(x2.reshape(n, nrhs), x) <= 1e-12
+
+
+def test_cholesky_solver_skips_on_failure(dev):
+    a = np.eye(40)
+    a[10, 10] = -1.0
+    b = np.arange(40, dtype=np.float64) + 1.0
+    _, x = _solve_gpu(a, b, dev, 3)
+    assert np.array_equal(x, b)
+
+
+def test_cholesky_invalid_arguments(dev):
+    import torch
+    import linearalgebra_amd as la
+    m = torch.eye(4, dtype=torch.float64, device=dev)
+    with pytest.raises(ValueError):
+        la.cholesky_decomposition.apply(None, 4)
+    with pytest.raises(ValueError):
+        la.cholesky_decomposition.apply(m, 0)
+    with pytest.raises(ValueError):
+        la.cholesky_decomposition.apply(m, 4, "5")
+    ws = torch.empty(8, dtype=torch.uint8, device=dev)
+    with pytest.raises(ValueError):
+        la.cholesky_decomposition.apply(m, 4, workspace=ws)
