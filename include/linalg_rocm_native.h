@@ -127,9 +127,12 @@ LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, voi
  * row order, negated for an odd number of interchanges, 0 when the factorisation finds a
  * zero pivot: the GPU reference's result. The CPU reference (src/matrix_operation/cpu/
  * determinant.cpp:19-28) omits the interchange sign (determinant.h:35-36), so the two agree
- * up to sign. Argument errors: -1 m, -2 pivot, -3 lda, -4 type, -5 workspace, -7 det.
- * Cholesky modes return LRN_E_UNSUPPORTED. SYNCHRONOUS on stream (the value is returned on
- * the host, as the reference's apply returns it). */
+ * up to sign. Cholesky modes: CHOLESKY factors m as lrn_cholesky_decomposition (LOWER) and
+ * CHOLESKY_UPFRONT reads the caller's lower factor; *det = (product of L's diagonal)^2, the GPU
+ * reference's value (inversion_hip.hip:62-80; the CPU reference returns the unsquared product),
+ * 0 when the factorisation fails. Argument errors: -1 m, -2 pivot, -3 lda, -4 type,
+ * -5 workspace, -7 det. SYNCHRONOUS on stream (the value is returned on the host, as the
+ * reference's apply returns it). */
 LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_determinant_workspace_size_f32(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_determinant_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,
@@ -142,9 +145,11 @@ LRN_API int lrn_determinant_f32(float* m, int64_t* pivot, int64_t lda, int type,
  * 13-53; CPU version src/matrix_operation/cpu/native/inversion_native.cpp:229-304,411-436).
  * m (row-major lda x lda) is overwritten with A^-1. LU mode factors first (pivots as for
  * the determinant) and, on a zero pivot, leaves the factors in m; LU_UPFRONT inverts the
- * caller's factors and pivots. Argument errors: -1 m, -2 pivot, -3 lda, -4 type,
- * -5 workspace. Cholesky modes return LRN_E_UNSUPPORTED. Workspace holds one lda x lda
- * matrix besides the factorisation's. SYNCHRONOUS on stream (it reports the panel's status). */
+ * caller's factors and pivots. Cholesky modes: A^-1 = L^-T L^-1 from the lower factor (CHOLESKY
+ * factors first and leaves m incomplete on failure; CHOLESKY_UPFRONT reads the caller's lower
+ * factor); the whole symmetric inverse is written, so its lower triangle is the reference's
+ * result. Argument errors: -1 m, -2 pivot, -3 lda, -4 type, -5 workspace. Workspace holds one
+ * lda x lda matrix besides the factorisation's. SYNCHRONOUS on stream. */
 LRN_API int lrn_inversion_workspace_size_f64(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_inversion_workspace_size_f32(int64_t lda, int type, size_t* bytes);
 LRN_API int lrn_inversion_f64(double* m, int64_t* pivot, int64_t lda, int type, void* workspace,

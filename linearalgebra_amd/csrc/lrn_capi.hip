@@ -1256,9 +1256,14 @@ int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t l
 // src/matrix_operation/gpu/hipblas/inversion_hip.hip:662-700), into pivots carved from the
 // workspace; LU_UPFRONT reads the caller's factors and pivots.
 bool is_lu_type(int type) { return type == LRN_LU_LINEAR_SOLVER || type == LRN_LU_UPFRONT_LINEAR_SOLVER; }
+bool is_chol_type(int type)
+{
+    return type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER;
+}
 
 template <typename T> struct ConsumerLayout {
     GetrfLayout<T> F;
+    CholLayout<T> C;
     long long* piv = nullptr;
     int* info = nullptr;
     T* result = nullptr;
@@ -1271,6 +1276,7 @@ template <typename T> ConsumerLayout<T> carve_consumer(Arena& ar, long long n, i
         L.F = carve_getrf<T>(ar);
         L.piv = ar.next<long long>(static_cast<size_t>(n));
     }
+    if (type == LRN_CHOLESKY_LINEAR_SOLVER) L.C = carve_chol<T>(ar, n);
     L.info = ar.next<int>(64);
     L.result = ar.next<T>(1);
     return L;
@@ -1292,30 +1298,33 @@ template <typename T> size_t inverse_ws_bytes(long long n, int type)
     return ar.off;
 }
 
-// LU X = X in place for all n columns of the n x n block X (row stride n), blocked like the
-// factorisation: 64-row strips solved by strip_trsm_kernel, the rest of each 256-row block
+// LU X = X in place for all n columns of the n x n block X (row stride n; L unit lower, or
+// non-unit lower for a Cholesky factor stored with L^T above it), blocked like the factorisation: 64-row strips solved by strip_trsm_kernel, the rest of each 256-row block
 // updated by a GEMM of depth 64, the rows beyond the block by one GEMM of depth 256.
 constexpr long long kInvNB = 256;
 
-template <typename T> int launch_strip(const T* LU, long long n, long long r0, int w, T* X, bool lower, hipStream_t s)
+template <typename T>
+int launch_strip(const T* LU, long long n, long long r0, int w, T* X, bool lower, hipStream_t s, bool unit = true)
 {
     const dim3 grid(static_cast<unsigned>((n + 63) / 64));
-    if (lower)
-        hipLaunchKernelGGL((strip_trsm_kernel<T, true>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
+    if (lower && unit)
+        hipLaunchKernelGGL((strip_trsm_kernel<T, true, true>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
+    else if (lower)
+        hipLaunchKernelGGL((strip_trsm_kernel<T, true, false>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
     else
         hipLaunchKernelGGL((strip_trsm_kernel<T, false>), grid, dim3(64), 0, s, LU, n, r0, w, X, n, n);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
 
-template <typename T> int blocked_lu_solve_all(const T* LU, long long n, T* X, hipStream_t s)
+template <typename T> int blocked_lu_solve_all(const T* LU, long long n, T* X, hipStream_t s, bool unit_lower = true)
 {
     // forward: unit lower L, top to bottom
     for (long long b0 = 0; b0 < n; b0 += kInvNB) {
         const long long b1 = std::min(n, b0 + kInvNB);
         for (long long s0 = b0; s0 < b1; s0 += kStripRows) {
             const long long s1 = std::min(b1, s0 + kStripRows);
-            LRN_TRY(launch_strip<T>(LU, n, s0, static_cast<int>(s1 - s0), X, true, s));
+            LRN_TRY(launch_strip<T>(LU, n, s0, static_cast<int>(s1 - s0), X, true, s, unit_lower));
             LRN_TRY(launch_gemm_sub<T>(LU + s1 * n + s0, n, X + s0 * n, n, X + s1 * n, n, b1 - s1, n, s1 - s0, s));
         }
         LRN_TRY(launch_gemm_sub<T>(LU + b1 * n + b0, n, X + b0 * n, n, X + b1 * n, n, n - b1, n, b1 - b0, s));
@@ -1333,12 +1342,28 @@ template <typename T> int blocked_lu_solve_all(const T* LU, long long n, T* X, h
     return LRN_OK;
 }
 
-// Factor (LU mode) and return the pivots to use; info ends up in L.info on the device.
+// Factor (LU or Cholesky mode) and return the pivots to use (none for Cholesky); info ends up
+// in L.info on the device. chol_layout: the Cholesky factor's layout pass (0 = L with zeros
+// above, 2 = L with L^T mirrored above), applied to the caller's factor in CHOLESKY_UPFRONT.
 template <typename T>
 int consumer_factor(T* m, int64_t* pivot, long long n, int type, const ConsumerLayout<T>& L, hipStream_t s,
-    const long long** piv_out)
+    const long long** piv_out, int chol_layout = 0)
 {
     LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.info), 0u, 64, s));
+    if (is_chol_type(type)) {
+        *piv_out = nullptr;
+        if (type == LRN_CHOLESKY_LINEAR_SOLVER) {
+            LRN_TRY(chol_factor<T>(m, n, chol_layout, L.C, s));
+            LRN_HIP(hipMemcpyAsync(L.info, L.C.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+        } else if (chol_layout == 2) {
+            const long long pairs = n * (n - 1) / 2;
+            const unsigned grid =
+                static_cast<unsigned>(std::max<long long>(1, std::min<long long>((pairs + 255) / 256, 1ll << 16)));
+            hipLaunchKernelGGL(chol_layout_kernel<T>, dim3(grid), dim3(256), 0, s, m, n, 2, L.info);
+            LRN_HIP(hipGetLastError());
+        }
+        return LRN_OK;
+    }
     if (type == LRN_LU_LINEAR_SOLVER) {
         long long* piv = pivot != nullptr ? reinterpret_cast<long long*>(pivot) : L.piv;
         LRN_TRY(getrf_dispatch<T>(m, n, piv, L.F.ws, s));
@@ -1353,9 +1378,8 @@ int consumer_factor(T* m, int64_t* pivot, long long n, int type, const ConsumerL
 // Status of the cooperative panel's bounded waits (LU mode), read after a synchronisation.
 template <typename T> int consumer_status(const ConsumerLayout<T>& L, int type, hipStream_t s)
 {
-    if (type != LRN_LU_LINEAR_SOLVER) return LRN_OK;
     int err = 0;
-    LRN_HIP(hipMemcpyAsync(&err, L.F.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (type == LRN_LU_LINEAR_SOLVER) LRN_HIP(hipMemcpyAsync(&err, L.F.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
     LRN_HIP(hipStreamSynchronize(s));
     return err != 0 ? LRN_E_TIMEOUT : LRN_OK;
 }
@@ -1366,8 +1390,7 @@ int determinant_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspac
 {
     if (m == nullptr) return -1;
     if (lda <= 0) return -3;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!is_lu_type(type)) return -4;
+    if (!is_lu_type(type) && !is_chol_type(type)) return -4;
     if (type == LRN_LU_UPFRONT_LINEAR_SOLVER && pivot == nullptr) return -2;
     if (det == nullptr) return -7;
     if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
@@ -1378,7 +1401,7 @@ int determinant_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspac
     const long long* piv = nullptr;
     LRN_TRY(consumer_factor<T>(m, pivot, lda, type, L, s, &piv));
     hipLaunchKernelGGL(lu_det_kernel<T>, dim3(1), dim3(256), 0, s, m, static_cast<long long>(lda),
-        static_cast<long long>(lda), piv, L.info, L.result);
+        static_cast<long long>(lda), piv, L.info, L.result, is_chol_type(type) ? 1 : 0);
     LRN_HIP(hipGetLastError());
     T host = T(0);
     LRN_HIP(hipMemcpyAsync(&host, L.result, sizeof(T), hipMemcpyDeviceToHost, s));
@@ -1396,8 +1419,7 @@ int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace,
 {
     if (m == nullptr) return -1;
     if (lda <= 0) return -3;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!is_lu_type(type)) return -4;
+    if (!is_lu_type(type) && !is_chol_type(type)) return -4;
     if (type == LRN_LU_UPFRONT_LINEAR_SOLVER && pivot == nullptr) return -2;
     if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
     if (workspace == nullptr || ws_size < inverse_ws_bytes<T>(lda, type)) return -5;
@@ -1408,12 +1430,18 @@ int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace,
     const auto G = carve_getrs<T>(ar, n, 1);
     T* X = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(n));
     const long long* piv = nullptr;
-    LRN_TRY(consumer_factor<T>(m, pivot, n, type, L, s, &piv));
+    const bool chol = is_chol_type(type);
+    LRN_TRY(consumer_factor<T>(m, pivot, n, type, L, s, &piv, 2));
     const unsigned grid = static_cast<unsigned>(std::min<long long>((n * n + 255) / 256, 1ll << 16));
-    LRN_TRY(getrs_perm<T>(n, piv, G, s));
-    hipLaunchKernelGGL(perm_eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n, G.perm);
+    if (chol) {
+        // L L^T X = I with L^T mirrored into the upper triangle
+        hipLaunchKernelGGL(eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n);
+    } else {
+        LRN_TRY(getrs_perm<T>(n, piv, G, s));
+        hipLaunchKernelGGL(perm_eye_kernel<T>, dim3(grid), dim3(256), 0, s, X, n, n, G.perm);
+    }
     LRN_HIP(hipGetLastError());
-    LRN_TRY(blocked_lu_solve_all<T>(m, n, X, s));
+    LRN_TRY(blocked_lu_solve_all<T>(m, n, X, s, !chol));
     hipLaunchKernelGGL(copy_unless_kernel<T>, dim3(grid), dim3(256), 0, s, X, m, n * n, L.info);
     LRN_HIP(hipGetLastError());
     return consumer_status<T>(L, type, s);
@@ -2007,8 +2035,7 @@ LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, voi
 LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes)
 {
     if (lda <= 0) return -1;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!lrn::is_lu_type(type)) return -2;
+    if (!lrn::is_lu_type(type) && !lrn::is_chol_type(type)) return -2;
     if (bytes == nullptr) return -3;
     *bytes = lrn::det_ws_bytes<double>(lda, type);
     return LRN_OK;
@@ -2016,8 +2043,7 @@ LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* by
 LRN_API int lrn_determinant_workspace_size_f32(int64_t lda, int type, size_t* bytes)
 {
     if (lda <= 0) return -1;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!lrn::is_lu_type(type)) return -2;
+    if (!lrn::is_lu_type(type) && !lrn::is_chol_type(type)) return -2;
     if (bytes == nullptr) return -3;
     *bytes = lrn::det_ws_bytes<float>(lda, type);
     return LRN_OK;
@@ -2036,8 +2062,7 @@ LRN_API int lrn_determinant_f32(float* m, int64_t* pivot, int64_t lda, int type,
 LRN_API int lrn_inversion_workspace_size_f64(int64_t lda, int type, size_t* bytes)
 {
     if (lda <= 0) return -1;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!lrn::is_lu_type(type)) return -2;
+    if (!lrn::is_lu_type(type) && !lrn::is_chol_type(type)) return -2;
     if (bytes == nullptr) return -3;
     *bytes = lrn::inverse_ws_bytes<double>(lda, type);
     return LRN_OK;
@@ -2045,8 +2070,7 @@ LRN_API int lrn_inversion_workspace_size_f64(int64_t lda, int type, size_t* byte
 LRN_API int lrn_inversion_workspace_size_f32(int64_t lda, int type, size_t* bytes)
 {
     if (lda <= 0) return -1;
-    if (type == LRN_CHOLESKY_LINEAR_SOLVER || type == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return LRN_E_UNSUPPORTED;
-    if (!lrn::is_lu_type(type)) return -2;
+    if (!lrn::is_lu_type(type) && !lrn::is_chol_type(type)) return -2;
     if (bytes == nullptr) return -3;
     *bytes = lrn::inverse_ws_bytes<float>(lda, type);
     return LRN_OK;

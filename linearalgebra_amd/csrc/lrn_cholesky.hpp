@@ -101,9 +101,11 @@ __global__ __launch_bounds__(64) void chol_rows_kernel(T* __restrict__ A, long l
     }
 }
 
-// Final layout: lower = zero the strict upper triangle; upper = U[j][i] = L[i][j] and the
-// strict lower triangle zeroed (cholesky_decomposition_native.cpp:262-265,312-319). One thread
-// per pair (i > j), grid-stride. Skipped when *info is nonzero (the factors are incomplete).
+// Final layout: upper = 0, zero the strict upper triangle; upper = 1, U[j][i] = L[i][j] and the
+// strict lower triangle zeroed (cholesky_decomposition_native.cpp:262-265,312-319); upper = 2,
+// L^T copied into the strict upper triangle with L kept (the symmetric storage the inversion
+// solves with). One thread per pair (i > j), grid-stride. Skipped when *info is nonzero (the
+// factors are incomplete).
 template <typename T>
 __global__ __launch_bounds__(256) void chol_layout_kernel(T* __restrict__ A, long long n, int upper,
     const int* __restrict__ info)
@@ -117,7 +119,9 @@ __global__ __launch_bounds__(256) void chol_layout_kernel(T* __restrict__ A, lon
         while (i * (i - 1) / 2 > q) --i;
         while ((i + 1) * i / 2 <= q) ++i;
         const long long j = q - i * (i - 1) / 2;
-        if (upper) {
+        if (upper == 2) {
+            A[j * n + i] = A[i * n + j];
+        } else if (upper) {
             A[j * n + i] = A[i * n + j];
             A[i * n + j] = T(0);
         } else {

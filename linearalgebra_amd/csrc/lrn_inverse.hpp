@@ -13,9 +13,10 @@ namespace lrn {
 // odd number of interchanges (piv[i] != i + 1, 1-based), and zero when `info` reports a zero
 // pivot. One workgroup: 256 diagonal entries are staged through LDS per step and one lane
 // keeps the running product. piv may be null (no sign), info may be null (not checked).
+// square = 1: the Cholesky form, (prod of L's diagonal)^2 (inversion_hip.hip:62-80).
 template <typename T>
 __global__ __launch_bounds__(256) void lu_det_kernel(const T* __restrict__ LU, long long lda, long long n,
-    const long long* __restrict__ piv, const int* __restrict__ info, T* __restrict__ out)
+    const long long* __restrict__ piv, const int* __restrict__ info, T* __restrict__ out, int square)
 {
     __shared__ T diag[256];
     __shared__ int odd;
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(256) void lu_det_kernel(const T* __restrict__ LU, l
     __syncthreads();
     if (t == 0) {
         T r = odd ? -prod : prod;
+        if (square) r = r * r;
         if (info != nullptr && *info != 0) r = T(0);
         out[0] = r;
     }
@@ -62,10 +64,11 @@ constexpr int kStripRows = 64;
 
 // One row strip of a blocked triangular solve with many right-hand sides:
 // X[r0 : r0 + w, :] = D^-1 X[r0 : r0 + w, :] for the w x w diagonal block D = A[r0 : r0 + w,
-// r0 : r0 + w] (w <= 64), unit lower (kLower) or non-unit upper. One thread per column of X
+// r0 : r0 + w] (w <= 64), lower (kLower; unit diagonal when kUnit) or non-unit upper. One
+// thread per column of X
 // (64-thread workgroups, consecutive columns so the row accesses coalesce), the column's w
 // values in registers, D staged in LDS and read as broadcasts.
-template <typename T, bool kLower>
+template <typename T, bool kLower, bool kUnit = true>
 __global__ __launch_bounds__(64) void strip_trsm_kernel(const T* __restrict__ A, long long lda, long long r0, int w,
     T* __restrict__ X, long long ldx, long long ncols)
 {
@@ -82,11 +85,14 @@ __global__ __launch_bounds__(64) void strip_trsm_kernel(const T* __restrict__ A,
     for (int i = 0; i < kStripRows; ++i) x[i] = (i < w) ? X[(r0 + i) * ldx + col] : T(0);
     if constexpr (kLower) {
 #pragma unroll
-        for (int i = 1; i < kStripRows; ++i) {
+        for (int i = 0; i < kStripRows; ++i) {
             T acc = x[i];
 #pragma unroll
             for (int k = 0; k < i; ++k) acc -= tri[i][k] * x[k];
-            x[i] = acc;
+            if constexpr (kUnit)
+                x[i] = acc;
+            else
+                x[i] = (i < w) ? acc / tri[i][i] : T(0);
         }
     } else {
 #pragma unroll

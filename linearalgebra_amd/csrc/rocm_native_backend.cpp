@@ -6,8 +6,8 @@
 //                               src/matrix_operation/cpu/native/linear_solver_native.cpp:24-28
 //   apply returns info == 0     include/matrix_operation/lu_decomposition.h:30-36
 //   workspace_size per type     lu_decomposition_native.cpp:293-326, linear_solver_native.cpp:225-241
-//   Cholesky modes              linear_solver: the lower factor (linear_solver_native.cpp:177-189);
-//                               determinant / inversion: not provided, they throw std::invalid_argument
+//   Cholesky modes              the lower factor (linear_solver_native.cpp:177-189); determinant squares
+//                               the diagonal product as the GPU reference does
 #include <stdexcept>
 #include <string>
 
@@ -33,14 +33,7 @@ int solver_type(linalg::linear_solver_type t)
     return v;
 }
 
-// determinant and inversion: LU modes only in this backend
-int lu_type(linalg::linear_solver_type t)
-{
-    const int v = static_cast<int>(t);
-    if (v == LRN_CHOLESKY_LINEAR_SOLVER || v == LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER)
-        throw std::invalid_argument("ROCM_NATIVE: Cholesky modes of determinant and inversion are not provided");
-    return solver_type(t);
-}
+
 
 template <typename T> struct ops;
 template <> struct ops<double> {
@@ -195,14 +188,14 @@ T det_apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type, vo
     if (m == nullptr) throw std::invalid_argument("determinant<ROCM_NATIVE>: m must not be null");
     if (lda <= 0) throw std::invalid_argument("determinant<ROCM_NATIVE>: lda must be positive");
     T out = T(0);
-    check(ops<T>::det(m, reinterpret_cast<int64_t*>(pivot), lda, lu_type(type), ws, ws_size, &out, stream),
+    check(ops<T>::det(m, reinterpret_cast<int64_t*>(pivot), lda, solver_type(type), ws, ws_size, &out, stream),
         "determinant<ROCM_NATIVE>::apply");
     return out;
 }
 template <typename T> size_t det_ws(linalg_int lda, linear_solver_type type)
 {
     size_t bytes = 0;
-    check(ops<T>::det_ws(lda, lu_type(type), &bytes), "determinant<ROCM_NATIVE>::workspace_size");
+    check(ops<T>::det_ws(lda, solver_type(type), &bytes), "determinant<ROCM_NATIVE>::workspace_size");
     return bytes;
 }
 template <typename T>
@@ -210,13 +203,13 @@ void inv_apply(T* m, linalg_int* pivot, linalg_int lda, linear_solver_type type,
 {
     if (m == nullptr) throw std::invalid_argument("inversion<ROCM_NATIVE>: m must not be null");
     if (lda <= 0) throw std::invalid_argument("inversion<ROCM_NATIVE>: lda must be positive");
-    check(ops<T>::inv(m, reinterpret_cast<int64_t*>(pivot), lda, lu_type(type), ws, ws_size, stream),
+    check(ops<T>::inv(m, reinterpret_cast<int64_t*>(pivot), lda, solver_type(type), ws, ws_size, stream),
         "inversion<ROCM_NATIVE>::apply");
 }
 template <typename T> size_t inv_ws(linalg_int lda, linear_solver_type type)
 {
     size_t bytes = 0;
-    check(ops<T>::inv_ws(lda, lu_type(type), &bytes), "inversion<ROCM_NATIVE>::workspace_size");
+    check(ops<T>::inv_ws(lda, solver_type(type), &bytes), "inversion<ROCM_NATIVE>::workspace_size");
     return bytes;
 }
 

@@ -5,9 +5,9 @@ Mirrors include/matrix_operation/determinant.h:13-55 and include/matrix_operatio
 caller-owned workspace). LU mode factors ``m`` first; LU_UPFRONT uses the caller's factors
 and pivots. The determinant carries the interchange sign, as the GPU reference's does
 (src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58); the CPU reference returns the
-unsigned diagonal product (src/matrix_operation/cpu/determinant.cpp:19-28). Cholesky modes
-raise ``LinalgError`` (unsupported). Every computation runs in the HIP library; there is no
-CPU fallback.
+unsigned diagonal product (src/matrix_operation/cpu/determinant.cpp:19-28). Cholesky modes use
+the lower factor; their determinant is the squared diagonal product, again the GPU reference's
+value. Every computation runs in the HIP library; there is no CPU fallback.
 """
 from __future__ import annotations
 

@@ -96,9 +96,9 @@ def test_argument_errors_before_device_work():
     assert lib.lrn_cholesky_decomposition_f64(dummy, 0, 0, dummy, 1 << 20, ctypes.byref(info), nil) == -2
     assert lib.lrn_cholesky_decomposition_f64(dummy, 4, 2, dummy, 1 << 20, ctypes.byref(info), nil) == -3
     assert lib.lrn_cholesky_decomposition_f64(dummy, 4, 0, dummy, 1, ctypes.byref(info), nil) == -4
-    # determinant / inversion: Cholesky modes not provided, unknown type, missing UPFRONT pivots
+    # determinant / inversion: too-small workspace, unknown type, missing UPFRONT pivots
     det = ctypes.c_double(0.0)
-    assert lib.lrn_determinant_f64(dummy, nil, 4, 3, dummy, 1 << 20, ctypes.byref(det), nil) == 3
+    assert lib.lrn_determinant_f64(dummy, nil, 4, 3, dummy, 1, ctypes.byref(det), nil) == -5
     assert lib.lrn_determinant_f64(dummy, nil, 4, 7, dummy, 1 << 20, ctypes.byref(det), nil) == -4
     assert lib.lrn_determinant_f64(dummy, nil, 4, 2, dummy, 1 << 20, ctypes.byref(det), nil) == -2
     assert lib.lrn_inversion_f64(dummy, nil, 4, 1, dummy, 1, nil) == -5

@@ -136,3 +136,58 @@ def test_cholesky_invalid_arguments(dev):
     ws = torch.empty(8, dtype=torch.uint8, device=dev)
     with pytest.raises(ValueError):
         la.cholesky_decomposition.apply(m, 4, workspace=ws)
+
+
+def _det(a, dev, type):
+    import torch
+    import linearalgebra_amd as la
+    n = a.shape[0]
+    m = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    return la.determinant.apply(m, None, n, la.linear_solver_type(type)), m.cpu().numpy()
+
+
+def _inv(a, dev, type):
+    import torch
+    import linearalgebra_amd as la
+    n = a.shape[0]
+    m = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    la.inversion.apply(m, None, n, la.linear_solver_type(type))
+    torch.cuda.synchronize()
+    return m.cpu().numpy()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_cholesky_determinant(dtype, tol, dev):
+    """The GPU reference squares the diagonal product (inversion_hip.hip:62-80): diag(4, 9) gives
+    36 and diag(1, 4, 16) 64 (TestDeterminant.cxx:108-126 checks result^2 on the CPU, whose value
+    is unsquared); the LU and Cholesky determinants of an SPD matrix agree
+    (TestGpuMatrixOperations.cxx:550-615); a non-SPD matrix gives 0."""
+    assert abs(_det(np.diag(np.array([4.0, 9.0], dtype=dtype)), dev, 3)[0] - 36) <= 36 * tol
+    assert abs(_det(np.diag(np.array([1.0, 4.0, 16.0], dtype=dtype)), dev, 3)[0] - 64) <= 64 * tol
+    a = O.inversion_spd_case(23, dtype, seed=23)
+    d_lu = _det(a, dev, 1)[0]
+    d_ch, lo = _det(a, dev, 3)
+    assert d_lu > 0 and abs(d_lu - d_ch) <= 50 * tol * abs(d_lu)
+    assert _det(lo, dev, 4)[0] == d_ch  # CHOLESKY_UPFRONT: the caller's factor
+    bad = np.eye(5, dtype=dtype)
+    bad[2, 2] = -1.0
+    assert _det(bad, dev, 3)[0] == 0
+
+
+@pytest.mark.parametrize("dim,dtype,tol", [(3, np.float64, 5e-12), (198, np.float64, 5e-12), (300, np.float64, 5e-12),
+                                           (3, np.float32, 2e-4), (198, np.float32, 2e-4)])
+def test_cholesky_inversion(dim, dtype, tol, dev):
+    """TestInversion.cxx:66-82: Cholesky-mode inverse of R R^T, its lower triangle mirrored,
+    times A within 5e-12 (FP64) / 2e-4 (FP32) of I at the reference's orders (198, 3; FP64 also
+    at 300); also the full result and CHOLESKY_UPFRONT."""
+    a = O.inversion_spd_case(dim, dtype, seed=dim + 1)
+    inv = _inv(a, dev, 3).astype(np.float64)
+    sym = np.tril(inv) + np.tril(inv, -1).T
+    eye = np.eye(dim)
+    err = np.abs(sym @ a.astype(np.float64) - eye).max()
+    print(f"cholesky inverse dim={dim} {np.dtype(dtype).name}: max|inv A - I| {err:.2e}", flush=True)
+    assert err <= tol
+    assert np.abs(inv @ a.astype(np.float64) - eye).max() <= tol
+    lo, _ = O.cholesky(a)
+    inv2 = _inv(lo, dev, 4).astype(np.float64)
+    assert np.abs(inv2 @ a.astype(np.float64) - eye).max() <= tol
