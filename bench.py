@@ -426,6 +426,40 @@ def cholesky_bench(torch, sizes, seed, dev, reps=2):
     return out
 
 
+def bmm_bench(torch, seed, dev, reps=5):
+    """batched_matrix_multiplication (FP64 and FP32) on count x (dim x dim) U(-5, 5) operands
+    for three shapes: median of `reps` HIP-event-timed launches after a warmup, the rate of
+    2 dim^3 count flops against the dense peak of the type, and the max error against torch."""
+    import linearalgebra_amd as la
+    out = []
+    for dim, count in ((9, 65536), (64, 16384), (256, 1024)):
+        for dt, name, peak in ((torch.float64, "f64", PEAK["fp64_mfma_tflops"]), (torch.float32, "f32", PEAK["fp32_tflops"])):
+            g = torch.Generator(device=dev)
+            g.manual_seed(seed + dim)
+            a = (torch.rand(count, dim, dim, dtype=dt, device=dev, generator=g) - 0.5) * 10
+            b = (torch.rand(count, dim, dim, dtype=dt, device=dev, generator=g) - 0.5) * 10
+            c = torch.empty_like(a)
+            times = []
+            for r in range(reps + 1):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                la.batched_matrix_multiplication(dim, count, a, b, c)
+                e1.record()
+                torch.cuda.synchronize()
+                if r > 0:
+                    times.append(e0.elapsed_time(e1))
+            ms = sorted(times)[len(times) // 2]
+            tf = 2.0 * dim ** 3 * count / (ms * 1e-3) / 1e12
+            err = float((c.double() - torch.bmm(a.double(), b.double())).abs().max())
+            out.append({"dim": dim, "count": count, "dtype": name, "ms": round(ms, 3), "tflops": round(tf, 2),
+                        "peak_frac": round(tf / peak, 4), "max_err": err})
+            log(f"bmm dim={dim} count={count} {name}: {ms:.3f} ms, {tf:.2f} TFLOP/s ({tf / peak:.3f} of peak), "
+                f"max err {err:.1e}")
+            del a, b, c
+    return out
+
+
 # ----------------------------------------------------------------------------- batched
 
 def batched_bench(torch, count, replays, seed, dev):
@@ -684,6 +718,7 @@ def main():
     triangular = None
     consumers = None
     cholesky = None
+    bmm = None
     if not args.no_configs:
         plain_configs = plain_configs_bench(torch, [int(v) for v in args.config_sizes.split(",") if v],
                                             (1, 64), args.seed, dev)
@@ -692,6 +727,7 @@ def main():
         triangular = triangular_bench(torch, n, args.seed, dev)
         consumers = consumers_bench(torch, [int(v) for v in args.consumer_sizes.split(",") if v], args.seed, dev)
         cholesky = cholesky_bench(torch, [int(v) for v in args.consumer_sizes.split(",") if v], args.seed, dev)
+        bmm = bmm_bench(torch, args.seed, dev)
         if args.mixed_extra_n > 0:
             mixed_configs += mixed_configs_bench(torch, [args.mixed_extra_n], [1e2, args.kappa] + [k for k in kappas if k > 1e6],
                                                  args.seed, dev, args.restart)
@@ -750,6 +786,7 @@ def main():
         "triangular_solver": triangular,
         "determinant_inversion": consumers,
         "cholesky": cholesky,
+        "batched_matrix_multiplication": bmm,
         "batched": batched,
         "batched_mixed": batched_mixed,
         "cpu_baseline": cpu,

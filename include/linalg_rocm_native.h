@@ -118,6 +118,17 @@ LRN_API int lrn_cholesky_decomposition_f64(double* c, int64_t lda, int upper, vo
 LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
     int* info, void* stream);
 
+/* ---- batched_matrix_multiplication -----------------------------------------------------
+ * Replaces batched_matrix_multiplication (include/matrix_operation/multiplication_batched.h:
+ * 7-42; ROCM_BLAS version src/matrix_operation/gpu/hipblas/multiplication_batched_hip.hip):
+ * C_i = A_i B_i for i < count, each dim x dim row-major, matrix i at offset i * dim * dim.
+ * Each output is an fma chain in increasing k. Argument errors, pointers first as the
+ * reference checks them: -3 a, -4 b, -5 c, then -1 dim, -2 count. ASYNCHRONOUS on stream. */
+LRN_API int lrn_batched_matrix_multiplication_f64(int64_t dim, int64_t count, const double* a, const double* b,
+    double* c, void* stream);
+LRN_API int lrn_batched_matrix_multiplication_f32(int64_t dim, int64_t count, const float* a, const float* b,
+    float* c, void* stream);
+
 /* ---- determinant<T, ROCM_NATIVE> -------------------------------------------------------
  * Replaces determinant<T, B>::apply / workspace_size (include/matrix_operation/determinant.h:
  * 13-55; ROCM_BLAS version src/matrix_operation/gpu/hipblas/inversion_hip.hip:36-58,440-482).

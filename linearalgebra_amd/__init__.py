@@ -18,6 +18,7 @@ from .lu import (  # noqa: F401
 from .mixed import gesv_mixed  # noqa: F401,E402
 from .triangular import triangular_solver  # noqa: F401,E402
 from .inverse import determinant, inversion  # noqa: F401,E402
+from .multiplication import batched_matrix_multiplication  # noqa: F401,E402
 
-__all__ = ["cholesky_decomposition", "cholesky_decomposition_enum", "determinant", "inversion", "gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type",
+__all__ = ["batched_matrix_multiplication", "cholesky_decomposition", "cholesky_decomposition_enum", "determinant", "inversion", "gesv_mixed", "lu_decomposition", "linear_solver", "linear_solver_type",
            "triangular_solver", "LinalgError", "set_tuning", "get_tuning"]

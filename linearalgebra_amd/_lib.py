@@ -31,6 +31,8 @@ SIGNATURES = {
     "lrn_triangular_solver_workspace_size_f32": [_i64, _i64, ctypes.POINTER(_sz)],
     "lrn_triangular_solver_f64": [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp],
     "lrn_triangular_solver_f32": [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp],
+    "lrn_batched_matrix_multiplication_f64": [_i64, _i64, _vp, _vp, _vp, _vp],
+    "lrn_batched_matrix_multiplication_f32": [_i64, _i64, _vp, _vp, _vp, _vp],
     "lrn_cholesky_decomposition_workspace_size_f64": [_i64, ctypes.POINTER(_sz)],
     "lrn_cholesky_decomposition_workspace_size_f32": [_i64, ctypes.POINTER(_sz)],
     "lrn_cholesky_decomposition_f64": [_vp, _i64, _int, _vp, _sz, ctypes.POINTER(_int), _vp],

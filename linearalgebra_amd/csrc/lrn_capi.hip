@@ -39,6 +39,7 @@
 #include "lrn_calu.hpp"
 #include "lrn_inverse.hpp"
 #include "lrn_cholesky.hpp"
+#include "lrn_bmm.hpp"
 
 namespace lrn {
 namespace {
@@ -1132,6 +1133,28 @@ int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, con
     return LRN_OK;
 }
 
+// ------------------------------------------------------------- batched multiplication
+// C_i = A_i B_i (include/matrix_operation/multiplication_batched.h:7-42): pointer checks first,
+// then the sizes (multiplication_batched_hip.hip:46-57); asynchronous on the stream.
+template <typename T>
+int batched_mm_impl(int64_t dim, int64_t count, const T* a, const T* b, T* c, void* stream)
+{
+    if (a == nullptr) return -3;
+    if (b == nullptr) return -4;
+    if (c == nullptr) return -5;
+    if (dim <= 0) return -1;
+    if (count <= 0) return -2;
+    if (dim > (1ll << 16)) return LRN_E_UNSUPPORTED;
+    const long long tn = (dim + kBmmTile - 1) / kBmmTile;
+    const long long tiles = tn * tn;
+    if (count > 0x7fffffffll / tiles) return LRN_E_UNSUPPORTED;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(batched_gemm_kernel<T>, dim3(static_cast<unsigned>(tiles * count)), dim3(256), 0, s,
+        static_cast<int>(dim), static_cast<long long>(count), static_cast<int>(tn), tiles, a, b, c);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
 template <typename T>
 int lu_decomposition_impl(T* m, int64_t lda, int64_t* pivot, void* workspace, size_t ws_size, int* info, void* stream)
 {
@@ -2005,6 +2028,17 @@ LRN_API int lrn_linear_solver_multi_f32(float* m, int64_t* pivot, int64_t lda, f
     int type, void* workspace, size_t workspace_size, void* stream)
 {
     return solver_impl<float>(m, pivot, lda, b, nrhs, ldb, type, workspace, workspace_size, stream);
+}
+
+LRN_API int lrn_batched_matrix_multiplication_f64(int64_t dim, int64_t count, const double* a, const double* b,
+    double* c, void* stream)
+{
+    return lrn::batched_mm_impl<double>(dim, count, a, b, c, stream);
+}
+LRN_API int lrn_batched_matrix_multiplication_f32(int64_t dim, int64_t count, const float* a, const float* b,
+    float* c, void* stream)
+{
+    return lrn::batched_mm_impl<float>(dim, count, a, b, c, stream);
 }
 
 LRN_API int lrn_cholesky_decomposition_workspace_size_f64(int64_t lda, size_t* bytes)

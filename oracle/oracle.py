@@ -468,3 +468,18 @@ def cholesky_known_answer(dim: int, dtype=np.float64, seed: int = 0):
         for j in range(i):
             c[i, j] = 0.5 * rng.random()
     return (c @ c.T).astype(dtype), c
+
+
+# ------------------------------------------------------------------- batched multiplication
+
+def batched_matmul(dim: int, count: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """C_i = A_i B_i the way batched_matmul_kernel does it
+    (src/matrix_operation/gpu/hipblas/multiplication_batched_hip.hip:12-37): every output an
+    accumulation in increasing k, here in float64 whatever the input type (the comparison
+    tolerance covers the working type's rounding)."""
+    A = np.asarray(a, dtype=np.float64).reshape(count, dim, dim)
+    B = np.asarray(b, dtype=np.float64).reshape(count, dim, dim)
+    C = np.zeros((count, dim, dim))
+    for k in range(dim):
+        C += A[:, :, k:k + 1] * B[:, k:k + 1, :]
+    return C
