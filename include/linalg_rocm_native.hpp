@@ -180,5 +180,12 @@ struct mixed_result
 size_t gesv_mixed_workspace_size(linalg_int n, int gmres_restart = 30, bool fallback = true);
 mixed_result gesv_mixed(const double* a, linalg_int n, const double* b, double* x, void* workspace,
     size_t workspace_size, double tol = 1e-14, int max_iter = 20, int gmres_restart = 30, void* stream = nullptr);
+// batched_matrix_multiplication (include/matrix_operation/multiplication_batched.h:7-42): the
+// reference declares it as a free function of namespace linalg; this backend's version lives
+// here so both can be linked into one program. C_i = A_i B_i, device pointers, asynchronous.
+void batched_matrix_multiplication(linalg_int dim, linalg_int count, const float* a, const float* b, float* c,
+    void* stream = nullptr);
+void batched_matrix_multiplication(linalg_int dim, linalg_int count, const double* a, const double* b, double* c,
+    void* stream = nullptr);
 }  // namespace rocm_native
 }  // namespace linalg

@@ -298,5 +298,16 @@ mixed_result gesv_mixed(const double* a, linalg_int n, const double* b, double* 
     return mixed_result{st.info, st.path, st.ir_iterations, st.gmres_iterations, st.backward_error};
 }
 
+void batched_matrix_multiplication(linalg_int dim, linalg_int count, const float* a, const float* b, float* c,
+    void* stream)
+{
+    check(lrn_batched_matrix_multiplication_f32(dim, count, a, b, c, stream), "batched_matrix_multiplication");
+}
+void batched_matrix_multiplication(linalg_int dim, linalg_int count, const double* a, const double* b, double* c,
+    void* stream)
+{
+    check(lrn_batched_matrix_multiplication_f64(dim, count, a, b, c, stream), "batched_matrix_multiplication");
+}
+
 }  // namespace rocm_native
 }  // namespace linalg

@@ -113,6 +113,10 @@ static void host_checks()
     EXPECT(throws<std::invalid_argument>([&] { DT::workspace_size(4, static_cast<linear_solver_type>(5)); }));
     EXPECT(throws<std::invalid_argument>([&] { IV::apply(nullptr, nullptr, 4, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
     EXPECT(throws<std::invalid_argument>([&] { IV::apply(m, nullptr, 4, static_cast<linear_solver_type>(5), ws, sizeof(ws)); }));
+    // batched multiplication: the reference's checks (pointers, then sizes) before any launch
+    EXPECT(throws<std::invalid_argument>(
+        [&] { linalg::rocm_native::batched_matrix_multiplication(2, 2, static_cast<const double*>(nullptr), m, m); }));
+    EXPECT(throws<std::invalid_argument>([&] { linalg::rocm_native::batched_matrix_multiplication(0, 2, m, m, m); }));
 }
 
 #define HIP_OK(x) EXPECT((x) == hipSuccess)
@@ -292,6 +296,23 @@ static void gpu_checks()
         EXPECT(det == 8.0);
         EXPECT(det_sw == -8.0);
         EXPECT(inv[0] == 0.25 && inv[1] == 0.0 && inv[2] == 0.0 && inv[3] == 0.5);
+        // batched multiplication of two 2 x 2 pairs: [[1,2],[3,4]] [[5,6],[7,8]] and I * B
+        const double ha[8] = {1, 2, 3, 4, 1, 0, 0, 1}, hb[8] = {5, 6, 7, 8, 2, 3, 4, 5};
+        double hc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
+        HIP_OK(hipMalloc(&d_a, sizeof(ha)));
+        HIP_OK(hipMalloc(&d_b, sizeof(hb)));
+        HIP_OK(hipMalloc(&d_c, sizeof(hc)));
+        HIP_OK(hipMemcpy(d_a, ha, sizeof(ha), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d_b, hb, sizeof(hb), hipMemcpyHostToDevice));
+        linalg::rocm_native::batched_matrix_multiplication(2, 2, d_a, d_b, d_c);
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemcpy(hc, d_c, sizeof(hc), hipMemcpyDeviceToHost));
+        EXPECT(hc[0] == 19 && hc[1] == 22 && hc[2] == 43 && hc[3] == 50);
+        EXPECT(hc[4] == 2 && hc[5] == 3 && hc[6] == 4 && hc[7] == 5);
+        (void)hipFree(d_a);
+        (void)hipFree(d_b);
+        (void)hipFree(d_c);
         (void)hipFree(d_m);
         (void)hipFree(d_w);
     }
