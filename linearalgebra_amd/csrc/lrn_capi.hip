@@ -1149,6 +1149,14 @@ int batched_mm_impl(int64_t dim, int64_t count, const T* a, const T* b, T* c, vo
     const long long tiles = tn * tn;
     if (count > 0x7fffffffll / tiles) return LRN_E_UNSUPPORTED;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dim <= kBmmSmall) {
+        const long long total = dim * dim * count;
+        if ((total + 255) / 256 > 0x7fffffffll) return LRN_E_UNSUPPORTED;
+        hipLaunchKernelGGL(batched_gemm_small_kernel<T>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+            s, static_cast<int>(dim), total, a, b, c);
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
     hipLaunchKernelGGL(batched_gemm_kernel<T>, dim3(static_cast<unsigned>(tiles * count)), dim3(256), 0, s,
         static_cast<int>(dim), static_cast<long long>(count), static_cast<int>(tn), tiles, a, b, c);
     LRN_HIP(hipGetLastError());

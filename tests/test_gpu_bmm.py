@@ -19,7 +19,8 @@ def _bmm(dim, count, a, b, dev):
     return tc.cpu().numpy().reshape(count, dim, dim)
 
 
-@pytest.mark.parametrize("dim,count", [(9, 6), (1, 3), (16, 5), (63, 2), (64, 4), (65, 3), (130, 2), (257, 1)])
+@pytest.mark.parametrize("dim,count", [(9, 6), (1, 3), (16, 5), (17, 4), (63, 2), (64, 4), (65, 3), (130, 2),
+                                       (257, 1), (5, 100000)])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 5e-6)])
 def test_batched_matmul_matches_oracle(dim, count, dtype, tol, dev):
     rng = np.random.default_rng(dim * 10 + count)
