@@ -83,11 +83,21 @@ __global__ __launch_bounds__(64) void chol_rows_kernel(T* __restrict__ A, long l
     }
     __syncthreads();
     if (t < rows) {
-        for (int j = 0; j < bs; ++j) {
-            T acc = tile[t][j];
-            for (int p = 0; p < j; ++p) acc = acc - tile[t][p] * l11[j][p];
-            tile[t][j] = acc / l11[j][j];
+        // the row in registers: the dependent chain reads only the broadcast l11 entries
+        T x[kCholNB];
+#pragma unroll
+        for (int p = 0; p < kCholNB; ++p) x[p] = tile[t][p];
+#pragma unroll
+        for (int j = 0; j < kCholNB; ++j) {
+            if (j < bs) {
+                T acc = x[j];
+#pragma unroll
+                for (int p = 0; p < j; ++p) acc = acc - x[p] * l11[j][p];
+                x[j] = acc / l11[j][j];
+            }
         }
+#pragma unroll
+        for (int p = 0; p < kCholNB; ++p) tile[t][p] = x[p];
     }
     __syncthreads();
     for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {

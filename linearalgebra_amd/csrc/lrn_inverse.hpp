@@ -84,15 +84,19 @@ __global__ __launch_bounds__(64) void strip_trsm_kernel(const T* __restrict__ A,
 #pragma unroll
     for (int i = 0; i < kStripRows; ++i) x[i] = (i < w) ? X[(r0 + i) * ldx + col] : T(0);
     if constexpr (kLower) {
+        // each row's body under a branch on (i < w), as in the upper form: with a select
+        // instead, the unit form is scheduled with the LDS reads hoisted and spills to scratch
 #pragma unroll
         for (int i = 0; i < kStripRows; ++i) {
-            T acc = x[i];
+            if (i < w) {
+                T acc = x[i];
 #pragma unroll
-            for (int k = 0; k < i; ++k) acc -= tri[i][k] * x[k];
-            if constexpr (kUnit)
-                x[i] = acc;
-            else
-                x[i] = (i < w) ? acc / tri[i][i] : T(0);
+                for (int k = 0; k < i; ++k) acc -= tri[i][k] * x[k];
+                if constexpr (kUnit)
+                    x[i] = acc;
+                else
+                    x[i] = acc / tri[i][i];
+            }
         }
     } else {
 #pragma unroll
