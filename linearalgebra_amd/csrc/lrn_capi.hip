@@ -1071,7 +1071,7 @@ template <typename T> int chol_factor(T* A, long long n, int upper, const CholLa
     const long long ldw = chol_ldw(n);
     for (long long k = 0; k < n; k += kCholNB) {
         const int bs = static_cast<int>(std::min<long long>(kCholNB, n - k));
-        hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(256), 0, s, A, n, k, bs, L.info);
+        hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(64), 0, s, A, n, k, bs, L.info);
         LRN_HIP(hipGetLastError());
         const long long m = n - k - bs;
         if (m <= 0) break;

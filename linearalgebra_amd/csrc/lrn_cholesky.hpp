@@ -10,54 +10,52 @@ namespace lrn {
 
 constexpr int kCholNB = 64;
 
-// Diagonal block A[k : k + bs, k : k + bs] (bs <= 64) factored in LDS, right-looking: at step
+// Diagonal block A[k : k + bs, k : k + bs] (bs <= 64) factored right-looking: at step
 // j the updated diagonal d = a_jj - sum_{p<j} l_jp^2 fails the factorisation when d <= 0
 // (the reference's test, src/matrix_operation/cpu/native/cholesky_decomposition_native.cpp:
 // 139-142; NaN passes it), *info = k + j + 1; otherwise l_jj = sqrt(d), the column below is
 // divided by it and the rest of the block's lower triangle takes the rank-1 update. Only the
 // lower triangle is read or written. Skipped when *info is already nonzero.
 template <typename T>
-__global__ __launch_bounds__(256) void chol_diag_kernel(T* __restrict__ A, long long lda, long long k, int bs,
+__global__ __launch_bounds__(64) void chol_diag_kernel(T* __restrict__ A, long long lda, long long k, int bs,
     int* __restrict__ info)
 {
-    __shared__ T blk[kCholNB][kCholNB + 1];
-    __shared__ int fail;
+    // one wavefront: thread i holds row i of the block in registers; per step the diagonal and
+    // then column j pass through LDS (the same operations, in the same order, as a
+    // right-looking update of the block in place)
+    __shared__ T col[kCholNB];
+    __shared__ T diag;
     if (*info != 0) return;
     const int t = threadIdx.x;
-    for (int idx = t; idx < kCholNB * kCholNB; idx += 256) {
-        const int i = idx / kCholNB, j = idx - i * kCholNB;
-        blk[i][j] = (i < bs && j <= i) ? A[(k + i) * lda + k + j] : T(0);
-    }
-    if (t == 0) fail = 0;
-    __syncthreads();
-    for (int j = 0; j < bs; ++j) {
-        const T d = blk[j][j];
-        if (!(d <= T(0))) {
+    T x[kCholNB];
+#pragma unroll
+    for (int c = 0; c < kCholNB; ++c) x[c] = (t < bs && c <= t) ? A[(k + t) * lda + k + c] : T(0);
+#pragma unroll
+    for (int j = 0; j < kCholNB; ++j) {
+        if (j < bs) {
+            if (t == j) diag = x[j];
+            __syncthreads();
+            const T d = diag;
+            if (d <= T(0)) {
+                if (t == 0) info[0] = static_cast<int>(k + j + 1);
+                return;
+            }
             const T l = sqrt(d);
+            if (t == j) x[j] = l;
+            if (t > j) x[j] = x[j] / l;
+            col[t] = x[j];
             __syncthreads();
-            if (t == 0) blk[j][j] = l;
-            for (int i = j + 1 + t; i < bs; i += 256) blk[i][j] = blk[i][j] / l;
+            // lower triangle of the rest: (t, c) with j < c <= t
+#pragma unroll
+            for (int c = j + 1; c < kCholNB; ++c)
+                if (c <= t && t < bs) x[c] = x[c] - x[j] * col[c];
             __syncthreads();
-            // lower triangle of the remaining block: (i, c) with j < c <= i < bs
-            const int rem = bs - j - 1;
-            for (int idx = t; idx < rem * rem; idx += 256) {
-                const int i = j + 1 + idx / rem, c = j + 1 + idx % rem;
-                if (c <= i) blk[i][c] = blk[i][c] - blk[i][j] * blk[c][j];
-            }
-            __syncthreads();
-        } else {
-            if (t == 0) {
-                fail = j + 1;
-                info[0] = static_cast<int>(k + j + 1);
-            }
-            __syncthreads();
-            break;
         }
     }
-    if (fail != 0) return;
-    for (int idx = t; idx < kCholNB * kCholNB; idx += 256) {
-        const int i = idx / kCholNB, j = idx - i * kCholNB;
-        if (i < bs && j <= i) A[(k + i) * lda + k + j] = blk[i][j];
+    if (t < bs) {
+#pragma unroll
+        for (int c = 0; c < kCholNB; ++c)
+            if (c <= t) A[(k + t) * lda + k + c] = x[c];
     }
 }
 
