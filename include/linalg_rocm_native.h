@@ -118,6 +118,17 @@ LRN_API int lrn_cholesky_decomposition_f64(double* c, int64_t lda, int upper, vo
 LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, void* workspace, size_t workspace_size,
     int* info, void* stream);
 
+/* Asynchronous form used by the C++ cholesky_decomposition<T, ROCM_NATIVE> specialisation:
+ * replaces the reference's GPU specialisation (src/matrix_operation/gpu/hipblas/
+ * cholesky_decomposition_hip.cpp:69-100,135-170), whose status is caller-owned DEVICE memory
+ * written on `stream` without a host synchronisation. Same factorisation and layout as
+ * lrn_cholesky_decomposition_*; *info_dev (device, required: -6 when NULL) receives 0 or the
+ * failing order. ASYNCHRONOUS on stream. */
+LRN_API int lrn_cholesky_decomposition_async_f64(double* c, int64_t lda, int upper, void* workspace,
+    size_t workspace_size, int* info_dev, void* stream);
+LRN_API int lrn_cholesky_decomposition_async_f32(float* c, int64_t lda, int upper, void* workspace,
+    size_t workspace_size, int* info_dev, void* stream);
+
 /* ---- batched_matrix_multiplication -----------------------------------------------------
  * Replaces batched_matrix_multiplication (include/matrix_operation/multiplication_batched.h:
  * 7-42; ROCM_BLAS version src/matrix_operation/gpu/hipblas/multiplication_batched_hip.hip):
@@ -342,6 +353,12 @@ enum lrn_trace_kind {
 LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity);
 LRN_API int lrn_trace_stop(void);
 LRN_API int lrn_trace_read(int kind, int64_t* launches, double* milliseconds, double* work);
+
+/* Polls each bounded inter-workgroup wait makes before it gives up and raises the error
+ * word (no reference counterpart; for tests of the error paths): limit < 0 restores the
+ * default (2^20), 0 makes every wait expire at once, so the factorisation reports
+ * LRN_E_TIMEOUT and the asynchronous solvers leave NaN in their outputs. Process-wide. */
+LRN_API int lrn_debug_set_spin_limit(int64_t limit);
 
 /* Version string of this library. */
 LRN_API const char* lrn_version(void);

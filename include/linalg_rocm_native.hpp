@@ -14,6 +14,8 @@
  *   include/matrix_operation/triangular_solver.h:20-38 triangular_solver<T, B>
  *   include/matrix_operation/determinant.h:13-55       determinant<T, B>
  *   include/matrix_operation/inversion.h:13-53         inversion<T, B>
+ *   include/matrix_operation/cholesky_decomposition.h:10-94  cholesky_decomposition_enum,
+ *                                                      cholesky_decomposition<T, B>
  *
  * Inside the reference tree, define LRN_WITH_REFERENCE_HEADERS and put the reference's
  * include root on the include path: the reference's own declarations are used and only the
@@ -38,6 +40,7 @@
 #include "include/matrix_operation/triangular_solver.h"
 #include "include/matrix_operation/determinant.h"
 #include "include/matrix_operation/inversion.h"
+#include "include/matrix_operation/cholesky_decomposition.h"
 #else
 
 #if !defined(linalg_int)
@@ -103,6 +106,21 @@ template <typename T, linalg_backend Backend> struct inversion
         size_t workspace_size, void* stream = nullptr);
     static size_t workspace_size(linalg_int lda, linear_solver_type type);
 };
+
+// include/matrix_operation/cholesky_decomposition.h:15-19,38-94
+enum class cholesky_decomposition_enum : char
+{
+    LOWER_TRIANGULAR = 'L',
+    UPPER_TRIANGULAR = 'U'
+};
+
+template <typename T, linalg_backend Backend> struct cholesky_decomposition
+{
+    static bool apply(T* C, linalg_int lda, cholesky_decomposition_enum type, void* workspace, size_t workspace_size,
+        int* info, void* stream = nullptr);
+    static bool apply_aad(T* L_aad, const T* L, linalg_int lda, cholesky_decomposition_enum type, T* A_aad);
+    static size_t workspace_size(linalg_int lda, cholesky_decomposition_enum type);
+};
 }  // namespace linalg
 #endif
 
@@ -157,6 +175,19 @@ template <> size_t inversion<double, linalg_backend::ROCM_NATIVE>::workspace_siz
     linear_solver_type type);
 template <> size_t inversion<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
     linear_solver_type type);
+
+// cholesky_decomposition: as the reference's GPU specialisation, `info` is DEVICE memory (required)
+// written asynchronously on `stream`, and apply returns true once the work is enqueued (the
+// numerical status lives in *info). C is factored in place: L with zeros above (LOWER) or
+// U = L^T with zeros below (UPPER). apply_aad is not provided by this backend.
+template <> bool cholesky_decomposition<double, linalg_backend::ROCM_NATIVE>::apply(double* C, linalg_int lda,
+    cholesky_decomposition_enum type, void* workspace, size_t workspace_size, int* info, void* stream);
+template <> bool cholesky_decomposition<float, linalg_backend::ROCM_NATIVE>::apply(float* C, linalg_int lda,
+    cholesky_decomposition_enum type, void* workspace, size_t workspace_size, int* info, void* stream);
+template <> size_t cholesky_decomposition<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    cholesky_decomposition_enum type);
+template <> size_t cholesky_decomposition<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    cholesky_decomposition_enum type);
 
 // Extensions of this backend (no reference counterpart).
 namespace rocm_native

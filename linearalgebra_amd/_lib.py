@@ -37,6 +37,9 @@ SIGNATURES = {
     "lrn_cholesky_decomposition_workspace_size_f32": [_i64, ctypes.POINTER(_sz)],
     "lrn_cholesky_decomposition_f64": [_vp, _i64, _int, _vp, _sz, ctypes.POINTER(_int), _vp],
     "lrn_cholesky_decomposition_f32": [_vp, _i64, _int, _vp, _sz, ctypes.POINTER(_int), _vp],
+    "lrn_cholesky_decomposition_async_f64": [_vp, _i64, _int, _vp, _sz, _vp, _vp],
+    "lrn_cholesky_decomposition_async_f32": [_vp, _i64, _int, _vp, _sz, _vp, _vp],
+    "lrn_debug_set_spin_limit": [_i64],
     "lrn_determinant_workspace_size_f64": [_i64, _int, ctypes.POINTER(_sz)],
     "lrn_determinant_workspace_size_f32": [_i64, _int, ctypes.POINTER(_sz)],
     "lrn_determinant_f64": [_vp, _vp, _i64, _int, _vp, _sz, ctypes.POINTER(ctypes.c_double), _vp],

@@ -46,7 +46,10 @@ namespace {
 
 constexpr int kMaxW = 64;
 constexpr int kMaxInner = 16;  // inner panels per outer block column (row-map slots)
-constexpr unsigned kSpinLimit = 1u << 20;
+// Polls allowed to every bounded inter-workgroup wait (lrn_debug_set_spin_limit; 0 makes every
+// wait expire at once, which tests use to force the error paths).
+std::atomic<unsigned> g_spin_limit{1u << 20};
+unsigned spin_limit() { return g_spin_limit.load(std::memory_order_relaxed); }
 
 std::atomic<int> g_outer_nb{0};
 std::atomic<int> g_panel_w{0};
@@ -122,6 +125,7 @@ template <typename T> struct GetrsLayout {
     T* y;
     Chunk* xch;  // single right-hand side: tagged exchange of solved values, one per row
     int* maps;   // net row moves of every 64-pivot block (permutation from the pivots)
+    int* gate;   // nonzero: skip the solve (singular factors, or an expired factorisation wait)
 };
 
 long long solve_blocks(long long n) { return (n + kSolveTB - 1) / kSolveTB; }
@@ -137,6 +141,7 @@ template <typename T> GetrsLayout<T> carve_getrs(Arena& ar, long long n, long lo
     L.y = ar.next<T>(static_cast<size_t>(n) * static_cast<size_t>(std::min<long long>(nrhs, kSolveMaxCols)));
     L.xch = ar.next<Chunk>(static_cast<size_t>(n));
     L.maps = ar.next<int>(static_cast<size_t>((n + 63) / 64) * kPermMapInts);
+    L.gate = ar.next<int>(64);
     return L;
 }
 
@@ -266,7 +271,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
             if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
             TraceScope ts(kTrPanel, panel_flops(m, w), s);
             hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(P)), dim3(kPanelThreads), 0, s, A, lda, k, m, w, P, base,
-                kSpinLimit, piv, pw);
+                spin_limit(), piv, pw);
             LRN_HIP(hipGetLastError());
             base += static_cast<unsigned>(w);
             return LRN_OK;
@@ -297,7 +302,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
                 if (need_cus != nullptr) return need(P, per_cu);
                 if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
                 TraceScope ts(kTrPanel, panel_flops(m, w), s);
-                hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit,
+                hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, spin_limit(),
                     piv, pw);
                 LRN_HIP(hipGetLastError());
                 return LRN_OK;
@@ -336,7 +341,7 @@ int launch_panel(T* A, long long lda, long long k, long long m, int w, long long
         if (P > 1 && (per_cu <= 0 || static_cast<long long>(per_cu) * props.cus < P)) return LRN_E_UNSUPPORTED;
     }
     TraceScope ts(kTrPanel, panel_flops(m, w), s);
-    hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, kSpinLimit, piv, ws);
+    hipLaunchKernelGGL(kern, dim3(P), dim3(kPanelThreads), lds, s, A, lda, k, m, w, R, P, base, spin_limit(), piv, ws);
     LRN_HIP(hipGetLastError());
     if (P > 1) base += static_cast<unsigned>(P) * static_cast<unsigned>(w);
     return LRN_OK;
@@ -428,7 +433,7 @@ int launch_laswp_trsm(T* A, long long lda, const int* rowmap, int W2, long long 
 // C[M x N] -= A[M x K] * B[K x N]
 template <typename T>
 int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M,
-    long long N, long long K, hipStream_t s, int cus = 0)
+    long long N, long long K, hipStream_t s, int cus = 0, const int* skip = nullptr)
 {
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
     if constexpr (std::is_same<T, double>::value) {
@@ -451,11 +456,11 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<true>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd));
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
             else
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<false>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd));
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
             LRN_HIP(hipGetLastError());
             return LRN_OK;
         };
@@ -482,11 +487,11 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f32_kernel<true>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd));
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
             else
                 hipLaunchKernelGGL(gemm_sub_f32_kernel<false>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd));
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
             LRN_HIP(hipGetLastError());
             return LRN_OK;
         };
@@ -552,6 +557,30 @@ int launch_fill(unsigned* p, unsigned v, long long count, hipStream_t s)
 {
     if (count <= 0) return LRN_OK;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s, p, v, count);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+unsigned fill_grid(long long count)
+{
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>((count + 255) / 256, 4096)));
+}
+
+// x (n x nrhs, stride ldx) <- NaN when *err is raised (an expired substitution wait)
+template <typename T> int launch_poison(const int* err, T* x, long long n, long long nrhs, long long ldx, hipStream_t s)
+{
+    hipLaunchKernelGGL(poison_on_error_kernel<T>, dim3(fill_grid(n * nrhs)), dim3(256), 0, s, err, x, n, nrhs, ldx);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+// *gate <- singular or expired factorisation; x <- NaN on an expired wait
+template <typename T>
+int launch_gate(const int* info, const int* err, int* gate, T* x, long long n, long long nrhs, long long ldx,
+    hipStream_t s)
+{
+    hipLaunchKernelGGL(solve_gate_kernel<T>, dim3(fill_grid(n * nrhs)), dim3(256), 0, s, info, err, gate, x, n, nrhs,
+        ldx);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -836,10 +865,15 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     return LRN_OK;
 }
 
+// single_stream: the factorisation stays on `s` (no look-ahead helper streams, so no HIP object
+// is created); the results are bitwise those of the look-ahead order either way.
 template <typename T>
-int getrf_dispatch(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s)
+int getrf_dispatch(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, bool single_stream = false)
 {
     const int w = tuned_panel_w();
+    if (single_stream)
+        return w <= 32 ? getrf_blocked_plain<T, 32>(A, n, piv, ws, s, -1, nullptr)
+                       : getrf_blocked_plain<T, 64>(A, n, piv, ws, s, -1, nullptr);
     if (w <= 32) return getrf_blocked<T, 32>(A, n, piv, ws, s);
     return getrf_blocked<T, 64>(A, n, piv, ws, s);
 }
@@ -889,17 +923,19 @@ template <typename T> int getrs_perm(long long n, const long long* piv, const Ge
 // Forward and backward substitution with the permutation already in L.perm; factors of type
 // TF, right-hand sides and arithmetic of type T.
 // skip_dev: if non-null and nonzero, the kernels return without touching B.
+// sticky_err: L.err is neither cleared nor checked here (the caller clears it once and reads it
+// after a sequence of solves); otherwise it is cleared first and an expired wait leaves NaN in B.
 template <typename T, typename TF = T>
 int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& L,
-    const int* skip_dev, hipStream_t s)
+    const int* skip_dev, hipStream_t s, bool sticky_err = false)
 {
     const long long nblk = solve_blocks(n);
+    if (!sticky_err) LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         if (nc == 1) {
             // latency-optimised single right-hand side: tags 1 (forward) and 2 (backward)
             LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.xch), 0u, 4 * n, s));
-            LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
             const double tri_bytes = (0.5 * static_cast<double>(n) * n) * sizeof(TF) + 4.0 * n * sizeof(T);
             TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
             if constexpr (std::is_same<TF, float>::value && std::is_same<T, double>::value) {
@@ -912,16 +948,16 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
                         const unsigned nbk = static_cast<unsigned>((n + TB - 1) / TB);
                         if (vec4) {
                             hipLaunchKernelGGL((trsv_chain_kernel<T, true, true, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
-                                n, L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                                n, L.perm, B + c0, L.y, L.xch, 1u, spin_limit(), L.err, skip_dev);
                             LRN_HIP(hipGetLastError());
                             hipLaunchKernelGGL((trsv_chain_kernel<T, false, true, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
-                                n, L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                                n, L.perm, L.y, B + c0, L.xch, 2u, spin_limit(), L.err, skip_dev);
                         } else {
                             hipLaunchKernelGGL((trsv_chain_kernel<T, true, false, NS>), dim3(nbk), dim3(TB), 0, s, LU, n,
-                                n, L.perm, B + c0, L.y, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                                n, L.perm, B + c0, L.y, L.xch, 1u, spin_limit(), L.err, skip_dev);
                             LRN_HIP(hipGetLastError());
                             hipLaunchKernelGGL((trsv_chain_kernel<T, false, false, NS>), dim3(nbk), dim3(TB), 0, s, LU,
-                                n, n, L.perm, L.y, B + c0, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                                n, n, L.perm, L.y, B + c0, L.xch, 2u, spin_limit(), L.err, skip_dev);
                         }
                         LRN_HIP(hipGetLastError());
                         return LRN_OK;
@@ -934,25 +970,25 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
                 }
             }
             hipLaunchKernelGGL((trsv1_kernel<TF, T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n,
-                n, L.perm, B + c0, ldb, L.y, 1ll, L.xch, 1u, kSpinLimit, L.err, skip_dev);
+                n, L.perm, B + c0, ldb, L.y, 1ll, L.xch, 1u, spin_limit(), L.err, skip_dev);
             LRN_HIP(hipGetLastError());
             hipLaunchKernelGGL((trsv1_kernel<TF, T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n,
-                n, L.perm, L.y, 1ll, B + c0, ldb, L.xch, 2u, kSpinLimit, L.err, skip_dev);
+                n, L.perm, L.y, 1ll, B + c0, ldb, L.xch, 2u, spin_limit(), L.err, skip_dev);
             LRN_HIP(hipGetLastError());
             continue;
         }
         LRN_TRY(launch_fill(L.flags_f, 0u, nblk, s));
         LRN_TRY(launch_fill(L.flags_b, 0u, nblk, s));
-        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
         const double tri_bytes = (0.5 * static_cast<double>(n) * n + 2.0 * n * nc) * sizeof(T);
         TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
         hipLaunchKernelGGL((trsv_blocked_kernel<TF, T, true>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
-            L.perm, B + c0, ldb, L.y, static_cast<long long>(nc), nc, L.flags_f, kSpinLimit, L.err, skip_dev);
+            L.perm, B + c0, ldb, L.y, static_cast<long long>(nc), nc, L.flags_f, spin_limit(), L.err, skip_dev);
         LRN_HIP(hipGetLastError());
         hipLaunchKernelGGL((trsv_blocked_kernel<TF, T, false>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, LU, n, n,
-            L.perm, L.y, static_cast<long long>(nc), B + c0, ldb, nc, L.flags_b, kSpinLimit, L.err, skip_dev);
+            L.perm, L.y, static_cast<long long>(nc), B + c0, ldb, nc, L.flags_b, spin_limit(), L.err, skip_dev);
         LRN_HIP(hipGetLastError());
     }
+    if (!sticky_err) LRN_TRY(launch_poison<T>(L.err, B, n, nrhs, ldb, s));
     return LRN_OK;
 }
 
@@ -984,7 +1020,7 @@ int launch_trsm_chain(const T* a, long long lda, long long n, const T* b, long l
     unsigned* flags, int* err, hipStream_t s, const int* skip = nullptr)
 {
     hipLaunchKernelGGL((trsv_blocked_kernel<T, T, kFwd, kUnit, kTrans>), dim3(static_cast<unsigned>(solve_blocks(n))),
-        dim3(256), 0, s, a, lda, n, nullptr, b, ldb, x, ldx, nc, flags, kSpinLimit, err, skip);
+        dim3(256), 0, s, a, lda, n, nullptr, b, ldb, x, ldx, nc, flags, spin_limit(), err, skip);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -1011,10 +1047,10 @@ int triangular_solver_impl(int upper, int transpose, int unit, long long n, long
     const bool fwd = (upper != 0) == (transpose != 0);
     const bool u = unit != 0, t = transpose != 0;
     TraceScope ts(kTrTrsv, (0.5 * static_cast<double>(n) * n + 2.0 * static_cast<double>(n) * nrhs) * sizeof(T), s);
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         LRN_TRY(launch_fill(flags, 0u, solve_blocks(n), s));
-        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
         const T* bb = b + c0;
         T* xx = x + c0;
         int rc;
@@ -1030,7 +1066,8 @@ int triangular_solver_impl(int upper, int transpose, int unit, long long n, long
                         : launch_trsm_chain<T, false, false, false>(a, lda, n, bb, ldb, xx, ldx, nc, flags, err, s));
         LRN_TRY(rc);
     }
-    return LRN_OK;
+    // an expired wait between row blocks leaves NaN, never a wrong x
+    return launch_poison<T>(err, x, n, nrhs, ldx, s);
 }
 
 
@@ -1078,7 +1115,9 @@ template <typename T> int chol_factor(T* A, long long n, int upper, const CholLa
         hipLaunchKernelGGL(chol_rows_kernel<T>, dim3(static_cast<unsigned>((m + 63) / 64)), dim3(64), 0, s, A, n, k,
             bs, n, L.wt, ldw, L.info);
         LRN_HIP(hipGetLastError());
-        LRN_TRY(launch_gemm_sub<T>(A + (k + bs) * n + k, n, L.wt, ldw, A + (k + bs) * n + (k + bs), n, m, m, bs, s));
+        // skipped once a diagonal block has failed (the reference stops at the failing block)
+        LRN_TRY(launch_gemm_sub<T>(A + (k + bs) * n + k, n, L.wt, ldw, A + (k + bs) * n + (k + bs), n, m, m, bs, s, 0,
+            L.info));
     }
     const long long pairs = n * (n - 1) / 2;
     const unsigned grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>((pairs + 255) / 256, 1ll << 16)));
@@ -1094,6 +1133,25 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
     if (type == LRN_CHOLESKY_LINEAR_SOLVER) carve_chol<T>(ar, n);
     carve_getrs<T>(ar, n, nrhs);
     return ar.off;
+}
+
+// Asynchronous form: info to device memory, no host synchronisation (the reference's GPU
+// specialisation, src/matrix_operation/gpu/hipblas/cholesky_decomposition_hip.cpp:69-100).
+template <typename T>
+int cholesky_async_impl(T* c, int64_t lda, int upper, void* workspace, size_t ws_size, int* info_dev, void* stream)
+{
+    if (c == nullptr) return -1;
+    if (lda <= 0) return -2;
+    if (upper != 0 && upper != 1) return -3;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (workspace == nullptr || ws_size < chol_ws_bytes<T>(lda)) return -4;
+    if (info_dev == nullptr) return -6;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Arena ar(workspace, ws_size);
+    const auto L = carve_chol<T>(ar, lda);
+    LRN_TRY(chol_factor<T>(c, lda, upper, L, s));
+    LRN_HIP(hipMemcpyAsync(info_dev, L.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LRN_OK;
 }
 
 template <typename T>
@@ -1122,15 +1180,15 @@ template <typename T>
 int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& G,
     const int* skip, hipStream_t s)
 {
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         LRN_TRY(launch_fill(G.flags_f, 0u, solve_blocks(n), s));
         LRN_TRY(launch_fill(G.flags_b, 0u, solve_blocks(n), s));
-        LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
         LRN_TRY((launch_trsm_chain<T, true, false, false>(m, n, n, B + c0, ldb, G.y, nc, nc, G.flags_f, G.err, s, skip)));
         LRN_TRY((launch_trsm_chain<T, false, false, true>(m, n, n, G.y, nc, B + c0, ldb, nc, G.flags_b, G.err, s, skip)));
     }
-    return LRN_OK;
+    return launch_poison<T>(G.err, B, n, nrhs, ldb, s);
 }
 
 // ------------------------------------------------------------- batched multiplication
@@ -1244,7 +1302,7 @@ int getrf_tournament_impl(float* m, int64_t lda, int64_t* pivot, int* info_dev, 
 
 template <typename T>
 int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t ldb, int type, void* workspace,
-    size_t ws_size, void* stream)
+    size_t ws_size, void* stream, bool single_stream = false)
 {
     if (m == nullptr) return -1;
     if (lda <= 0) return -3;
@@ -1271,12 +1329,18 @@ int solver_impl(T* m, int64_t* pivot, int64_t lda, T* b, int64_t nrhs, int64_t l
         return chol_solve<T>(m, lda, b, nrhs, ldb, G, skip, s);
     }
     const int* skip = nullptr;
+    GetrfLayout<T> F{};
     if (type == LRN_LU_LINEAR_SOLVER) {
-        const auto F = carve_getrf<T>(ar);
-        LRN_TRY(getrf_dispatch<T>(m, lda, reinterpret_cast<long long*>(pivot), F.ws, s));
-        skip = F.ws.info;
+        F = carve_getrf<T>(ar);
+        LRN_TRY(getrf_dispatch<T>(m, lda, reinterpret_cast<long long*>(pivot), F.ws, s, single_stream));
     }
     const auto G = carve_getrs<T>(ar, lda, nrhs);
+    if (type == LRN_LU_LINEAR_SOLVER) {
+        // solve only nonsingular factors (linear_solver_native.cpp:160-171); a factorisation
+        // whose bounded waits expired leaves NaN in b instead of a wrong solution
+        LRN_TRY(launch_gate<T>(F.ws.info, F.ws.err, G.gate, b, lda, nrhs, ldb, s));
+        skip = G.gate;
+    }
     return getrs_run<T>(m, lda, reinterpret_cast<const long long*>(pivot), b, nrhs, ldb, G, skip, s);
 }
 
@@ -1473,7 +1537,8 @@ int inversion_impl(T* m, int64_t* pivot, int64_t lda, int type, void* workspace,
     }
     LRN_HIP(hipGetLastError());
     LRN_TRY(blocked_lu_solve_all<T>(m, n, X, s, !chol));
-    hipLaunchKernelGGL(copy_unless_kernel<T>, dim3(grid), dim3(256), 0, s, X, m, n * n, L.info);
+    hipLaunchKernelGGL(copy_unless_kernel<T>, dim3(grid), dim3(256), 0, s, X, m, n * n, L.info,
+        type == LRN_LU_LINEAR_SOLVER ? L.F.ws.err : nullptr);
     LRN_HIP(hipGetLastError());
     return consumer_status<T>(L, type, s);
 }
@@ -1595,7 +1660,8 @@ struct MixedSolver {
     int precond(const double* in, double* out)
     {
         if (in != out) LRN_TRY(copy(out, in));
-        return getrs_subst<double, float>(L.a32, n, out, 1, 1, L.g, nullptr, s);
+        // L.g.err is sticky over the whole refinement and read once at the end
+        return getrs_subst<double, float>(L.a32, n, out, 1, 1, L.g, nullptr, s, true);
     }
     // r = b - A x and the backward error
     int residual(double* berr)
@@ -1739,6 +1805,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     LRN_HIP(hipGetLastError());
     LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd, g_mixed_panel.load() ? &L.calu : nullptr)));
     LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.g.err), 0u, 64, s));
     LRN_HIP(hipEventRecord(e1, s));
     int info = 0, err = 0;
     LRN_HIP(hipMemcpyAsync(&info, L.f.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1818,16 +1885,26 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
             }
         }
     }
+    int fb_info = 0, fb_err = 0;
     if (!done && fallback) {
         LRN_HIP(hipMemcpyAsync(L.afb, A, sizeof(double) * static_cast<size_t>(nn), hipMemcpyDeviceToDevice, s));
         LRN_TRY(getrf_dispatch<double>(L.afb, n, L.pivfb, L.ffb.ws, s));
         LRN_TRY(S.copy(x, b));
-        LRN_TRY(getrs_run<double>(L.afb, n, L.pivfb, x, 1, 1, L.gfb, L.ffb.ws.info, s));
+        LRN_TRY(launch_gate<double>(L.ffb.ws.info, L.ffb.ws.err, L.gfb.gate, x, n, 1, 1, s));
+        LRN_TRY(getrs_run<double>(L.afb, n, L.pivfb, x, 1, 1, L.gfb, L.gfb.gate, s));
         LRN_TRY(S.residual(&berr));
+        LRN_HIP(hipMemcpyAsync(&fb_info, L.ffb.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+        LRN_HIP(hipMemcpyAsync(&fb_err, L.ffb.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
         st.path = 2;
     }
+    int subst_err = 0;
+    LRN_HIP(hipMemcpyAsync(&subst_err, L.g.err, sizeof(int), hipMemcpyDeviceToHost, s));
     LRN_HIP(hipEventRecord(e2, s));
     LRN_HIP(hipEventSynchronize(e2));
+    // an expired bounded wait anywhere (refinement substitutions, fallback factorisation or
+    // its solve) invalidates the result
+    if (subst_err != 0 || fb_err != 0) return LRN_E_TIMEOUT;
+    if (st.path == 2) st.info = fb_info;  // the status of the factors that produced x
     float f_ms = 0.f, r_ms = 0.f;
     LRN_HIP(hipEventElapsedTime(&f_ms, e0, e1));
     LRN_HIP(hipEventElapsedTime(&r_ms, e1, e2));
@@ -2072,6 +2149,17 @@ LRN_API int lrn_cholesky_decomposition_f32(float* c, int64_t lda, int upper, voi
     int* info, void* stream)
 {
     return lrn::cholesky_impl<float>(c, lda, upper, workspace, workspace_size, info, stream);
+}
+
+LRN_API int lrn_cholesky_decomposition_async_f64(double* c, int64_t lda, int upper, void* workspace,
+    size_t workspace_size, int* info_dev, void* stream)
+{
+    return cholesky_async_impl<double>(c, lda, upper, workspace, workspace_size, info_dev, stream);
+}
+LRN_API int lrn_cholesky_decomposition_async_f32(float* c, int64_t lda, int upper, void* workspace,
+    size_t workspace_size, int* info_dev, void* stream)
+{
+    return cholesky_async_impl<float>(c, lda, upper, workspace, workspace_size, info_dev, stream);
 }
 
 LRN_API int lrn_determinant_workspace_size_f64(int64_t lda, int type, size_t* bytes)
@@ -2326,6 +2414,13 @@ LRN_API int lrn_set_panel_kernel(int kind)
 {
     if (kind < 0 || kind > 1) return -1;
     g_panel_kind.store(kind);
+    return LRN_OK;
+}
+
+LRN_API int lrn_debug_set_spin_limit(int64_t limit)
+{
+    if (limit > 0x7fffffffll) return -1;
+    g_spin_limit.store(limit < 0 ? (1u << 20) : static_cast<unsigned>(limit));
     return LRN_OK;
 }
 

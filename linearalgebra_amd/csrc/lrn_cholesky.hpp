@@ -30,32 +30,37 @@ __global__ __launch_bounds__(64) void chol_diag_kernel(T* __restrict__ A, long l
     T x[kCholNB];
 #pragma unroll
     for (int c = 0; c < kCholNB; ++c) x[c] = (t < bs && c <= t) ? A[(k + t) * lda + k + c] : T(0);
+    int fail = kCholNB;  // step at which the factorisation failed (uniform: read from LDS)
 #pragma unroll
     for (int j = 0; j < kCholNB; ++j) {
-        if (j < bs) {
+        if (j < bs && fail == kCholNB) {
             if (t == j) diag = x[j];
             __syncthreads();
             const T d = diag;
             if (d <= T(0)) {
                 if (t == 0) info[0] = static_cast<int>(k + j + 1);
-                return;
-            }
-            const T l = sqrt(d);
-            if (t == j) x[j] = l;
-            if (t > j) x[j] = x[j] / l;
-            col[t] = x[j];
-            __syncthreads();
-            // lower triangle of the rest: (t, c) with j < c <= t
+                fail = j;
+            } else {
+                const T l = sqrt(d);
+                if (t == j) x[j] = l;
+                if (t > j) x[j] = x[j] / l;
+                col[t] = x[j];
+                __syncthreads();
+                // lower triangle of the rest: (t, c) with j < c <= t
 #pragma unroll
-            for (int c = j + 1; c < kCholNB; ++c)
-                if (c <= t && t < bs) x[c] = x[c] - x[j] * col[c];
-            __syncthreads();
+                for (int c = j + 1; c < kCholNB; ++c)
+                    if (c <= t && t < bs) x[c] = x[c] - x[j] * col[c];
+                __syncthreads();
+            }
         }
     }
-    if (t < bs) {
+    // on failure at step f: what the reference's row-by-row kernel has written when it stops at
+    // row f (cholesky_decomposition_native.cpp:124-142), i.e. the completed rows and row f's
+    // multipliers; row f's diagonal and the rows after it stay as they were
+    if (t < bs && t <= fail) {
 #pragma unroll
         for (int c = 0; c < kCholNB; ++c)
-            if (c <= t) A[(k + t) * lda + k + c] = x[c];
+            if (c <= t && c < fail) A[(k + t) * lda + k + c] = x[c];
     }
 }
 

@@ -75,14 +75,22 @@ __device__ __forceinline__ bool error_raised(const int* err)
 // `limit` polls it raises *err and returns, and once *err is raised every later wait
 // returns at once, so a kernel always exits even if the workgroups it waits for never
 // arrive (the host then reports the error instead of a result).
+__device__ __forceinline__ void raise_error(int* err) { __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// A wait polls at most `limit` times; limit 0 expires at once without polling (the host sets it
+// to force every wait's error path in tests).
 __device__ __forceinline__ void wait_counter(const unsigned* ctr, unsigned target, unsigned limit, int* err)
 {
+    if (limit == 0u) {
+        raise_error(err);
+        return;
+    }
     unsigned spins = 0;
     while (__hip_atomic_load(const_cast<unsigned*>(ctr), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
         ++spins;
         if ((spins & 255u) == 0u && error_raised(err)) break;
         if (spins > limit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            raise_error(err);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -95,12 +103,16 @@ __device__ __forceinline__ void wait_counter(const unsigned* ctr, unsigned targe
 // (an agent-scope acquire invalidates the XCD's L2 on every poll).
 __device__ __forceinline__ void wait_flag_coherent(const unsigned* flag, unsigned target, unsigned limit, int* err)
 {
+    if (limit == 0u) {
+        raise_error(err);
+        return;
+    }
     unsigned spins = 0;
     while (__hip_atomic_load(const_cast<unsigned*>(flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         ++spins;
         if ((spins & 255u) == 0u && error_raised(err)) break;
         if (spins > limit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            raise_error(err);
             break;
         }
         __builtin_amdgcn_s_sleep(1);

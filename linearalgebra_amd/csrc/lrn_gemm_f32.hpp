@@ -70,8 +70,9 @@ __device__ __forceinline__ void g32_load(g32v4 (&ra)[4], g32v4 (&rb)[4], const f
 template <bool kCheck>
 __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(const float* __restrict__ A, long long lda,
     const float* __restrict__ B, long long ldb, float* __restrict__ C, long long ldc, int M, int N, int K, int tm,
-    int tn, int per_xcd)
+    int tn, int per_xcd, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip != 0) return;  // e.g. a Cholesky factorisation that has failed
     constexpr int BM = kG32BM, BN = kG32BN, BK = kG32BK;
     constexpr int PAD = 4;
     typedef float acc_t __attribute__((ext_vector_type(4)));

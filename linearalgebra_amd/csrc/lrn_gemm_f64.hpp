@@ -107,8 +107,9 @@ __device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const d
 template <bool kCheck>
 __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __restrict__ A, long long lda,
     const double* __restrict__ B, long long ldb, double* __restrict__ C, long long ldc, int M, int N, int K, int tm,
-    int tn, int per_xcd)
+    int tn, int per_xcd, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip != 0) return;  // e.g. a Cholesky factorisation that has failed
     constexpr int BM = kG64BM, BN = kG64BN, BK = kG64BK;
     constexpr int PADA = 4, PADB = 4;
     constexpr int RG = 8, CG = 4;  // 4-row groups and 16-column groups per wave (32 x 64 of C)

@@ -211,6 +211,11 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel_kernel(T* __restric
             }
             __syncthreads();
             p = win_idx;
+            if (p < i || p >= k + m) {
+                // only an expired wait can get here: report it and keep the rows in place
+                if (tid == 0) raise_error(ws.err);
+                p = i;
+            }
             const int wp = static_cast<int>((p - k) / R);
             if (tid < w) {
                 prow[tid] = (wp == b) ? tile[static_cast<int>(p - row0) * LD + tid]
@@ -807,6 +812,34 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long count)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
+}
+
+// Gate of a solve that follows an asynchronous factorisation: *gate = 1 when the factors are
+// singular (*info != 0; the reference's LU mode then leaves x unchanged) or when a bounded wait
+// of the factorisation expired (*err != 0). In the second case the right-hand sides (n x nrhs,
+// row stride ldx) are also overwritten with NaN, so a timed-out factorisation can never be
+// read as a solution.
+template <typename T>
+__global__ void solve_gate_kernel(const int* __restrict__ info, const int* __restrict__ err, int* __restrict__ gate,
+    T* __restrict__ x, long long n, long long nrhs, long long ldx)
+{
+    const bool expired = *err != 0;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) *gate = (expired || *info != 0) ? 1 : 0;
+    if (!expired) return;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long e = i; e < n * nrhs; e += stride) x[(e / nrhs) * ldx + e % nrhs] = static_cast<T>(__builtin_nan(""));
+}
+
+// NaN over x (n x nrhs, row stride ldx) when a bounded wait of the substitution expired.
+template <typename T>
+__global__ void poison_on_error_kernel(const int* __restrict__ err, T* __restrict__ x, long long n, long long nrhs,
+    long long ldx)
+{
+    if (*err == 0) return;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < n * nrhs; e += stride)
+        x[(e / nrhs) * ldx + e % nrhs] = static_cast<T>(__builtin_nan(""));
 }
 
 }  // namespace lrn

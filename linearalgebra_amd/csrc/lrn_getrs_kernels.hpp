@@ -298,12 +298,13 @@ __global__ __launch_bounds__(256) void trsv1_kernel(const TF* __restrict__ LU, l
                 const Chunk* const pc[1] = {xch + g};
                 u32x4 c[1];
                 unsigned spins = 0;
+                if (spin_limit == 0u) raise_error(err);  // forced expiry (tests)
                 for (;;) {
                     ld_chunks<1>(c, pc);
                     if (c[0].w == tag) break;
                     ++spins;
                     if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        raise_error(err);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -478,12 +479,13 @@ __global__ __launch_bounds__(kChainSB * NS) void trsv_chain_kernel(const float* 
                 const Chunk* const pc[1] = {xch + g};
                 u32x4 c[1];
                 unsigned spins = 0;
+                if (spin_limit == 0u) raise_error(err);  // forced expiry (tests)
                 for (;;) {
                     ld_chunks<1>(c, pc);
                     if (c[0].w == tag) break;
                     ++spins;
                     if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        raise_error(err);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);

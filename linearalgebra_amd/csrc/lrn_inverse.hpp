@@ -129,9 +129,10 @@ __global__ __launch_bounds__(256) void eye_kernel(T* __restrict__ X, long long n
 // leaves the factors in place, as the reference's LU inversion does); grid-stride loop.
 template <typename T>
 __global__ __launch_bounds__(256) void copy_unless_kernel(const T* __restrict__ src, T* __restrict__ dst,
-    long long count, const int* __restrict__ skip)
+    long long count, const int* __restrict__ skip, const int* __restrict__ err)
 {
     if (skip != nullptr && *skip != 0) return;
+    if (err != nullptr && *err != 0) return;  // a bounded wait of the factorisation expired
     const long long stride = static_cast<long long>(gridDim.x) * 256;
     for (long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; idx < count; idx += stride)
         dst[idx] = src[idx];

@@ -132,6 +132,12 @@ template <int N>
 __device__ __forceinline__ void spin_chunks(u32x4 (&r)[N], const Chunk* const (&p)[N], unsigned tag, unsigned spin_limit,
     int* err)
 {
+    if (spin_limit == 0u) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) r[s] = u32x4{0u, 0u, 0u, 0u};
+        raise_error(err);
+        return;
+    }
     unsigned spins = 0;
     for (;;) {
         ld_chunks<N>(r, p);
@@ -142,7 +148,7 @@ __device__ __forceinline__ void spin_chunks(u32x4 (&r)[N], const Chunk* const (&
         if (ok) return;
         ++spins;
         if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            raise_error(err);
             return;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -151,6 +157,11 @@ __device__ __forceinline__ void spin_chunks(u32x4 (&r)[N], const Chunk* const (&
 
 __device__ __forceinline__ void spin_chunk1(u32x4& r, const Chunk* p, unsigned tag, unsigned spin_limit, int* err)
 {
+    if (spin_limit == 0u) {
+        r = u32x4{0u, 0u, 0u, 0u};
+        raise_error(err);
+        return;
+    }
     unsigned spins = 0;
     for (;;) {
         const Chunk* const pc[1] = {p};
@@ -160,7 +171,7 @@ __device__ __forceinline__ void spin_chunk1(u32x4& r, const Chunk* p, unsigned t
         if (r.w == tag) return;
         ++spins;
         if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(err))) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            raise_error(err);
             return;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -329,6 +340,7 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
             // (v) gather all candidates and the diagonal row
             u32x4 r[CPT];
             unsigned spins = 0;
+            if (spin_limit == 0u) raise_error(ws.err);  // forced expiry (tests)
             for (;;) {
                 if (par)
                     ld_chunks<CPT>(r, addr1);
@@ -341,7 +353,7 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
                 if (ok) break;
                 ++spins;
                 if (spins > spin_limit || ((spins & 255u) == 0u && error_raised(ws.err))) {
-                    __hip_atomic_store(ws.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    raise_error(ws.err);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -369,6 +381,11 @@ __global__ __launch_bounds__(kPanelThreads) void getf2_panel3_kernel(T* __restri
                 }
                 wave_argmax_ordered(gv, gi);  // lane b holds workgroup b's row range
                 if (lane == 0) {
+                    if (gi < i || gi >= k + m) {
+                        // only an expired wait can get here: report it, keep the rows in place
+                        raise_error(ws.err);
+                        gi = i;
+                    }
                     win_row = gi;
                     win_b = static_cast<int>((gi - k) / R);
                 }

@@ -8,6 +8,8 @@
 //   workspace_size per type     lu_decomposition_native.cpp:293-326, linear_solver_native.cpp:225-241
 //   Cholesky modes              the lower factor (linear_solver_native.cpp:177-189); determinant squares
 //                               the diagonal product as the GPU reference does
+//   cholesky_decomposition      device info, asynchronous (src/matrix_operation/gpu/hipblas/
+//                               cholesky_decomposition_hip.cpp:69-100,135-170)
 #include <stdexcept>
 #include <string>
 
@@ -49,6 +51,8 @@ template <> struct ops<double> {
     static constexpr auto det = lrn_determinant_f64;
     static constexpr auto inv_ws = lrn_inversion_workspace_size_f64;
     static constexpr auto inv = lrn_inversion_f64;
+    static constexpr auto chol_ws = lrn_cholesky_decomposition_workspace_size_f64;
+    static constexpr auto chol = lrn_cholesky_decomposition_async_f64;
 };
 template <> struct ops<float> {
     static constexpr auto lu_ws = lrn_lu_decomposition_workspace_size_f32;
@@ -63,6 +67,8 @@ template <> struct ops<float> {
     static constexpr auto det = lrn_determinant_f32;
     static constexpr auto inv_ws = lrn_inversion_workspace_size_f32;
     static constexpr auto inv = lrn_inversion_f32;
+    static constexpr auto chol_ws = lrn_cholesky_decomposition_workspace_size_f32;
+    static constexpr auto chol = lrn_cholesky_decomposition_async_f32;
 };
 
 template <typename T> bool lu_apply(T* m, linalg_int lda, linalg_int* pivot, void* ws, size_t ws_size, int* info, void* stream)
@@ -252,6 +258,54 @@ template <> size_t inversion<float, linalg_backend::ROCM_NATIVE>::workspace_size
     linear_solver_type type)
 {
     return inv_ws<float>(lda, type);
+}
+
+// cholesky_decomposition: the reference's GPU contract (src/matrix_operation/gpu/hipblas/
+// cholesky_decomposition_hip.cpp:69-100,135-170): device info required, asynchronous, true once
+// enqueued; the triangle is chosen by the enum ('L' / 'U').
+template <typename T>
+bool chol_apply(T* C, linalg_int lda, cholesky_decomposition_enum type, void* ws, size_t ws_size, int* info,
+    void* stream)
+{
+    if (info == nullptr) throw std::invalid_argument("cholesky_decomposition<ROCM_NATIVE>: device info must not be null");
+    if (C == nullptr) throw std::invalid_argument("cholesky_decomposition<ROCM_NATIVE>: C must not be null");
+    if (lda <= 0) throw std::invalid_argument("cholesky_decomposition<ROCM_NATIVE>: lda must be positive");
+    int upper = 0;
+    if (type == cholesky_decomposition_enum::UPPER_TRIANGULAR)
+        upper = 1;
+    else if (type != cholesky_decomposition_enum::LOWER_TRIANGULAR)
+        throw std::invalid_argument("cholesky_decomposition<ROCM_NATIVE>: unknown triangle");
+    check(ops<T>::chol(C, lda, upper, ws, ws_size, info, stream), "cholesky_decomposition<ROCM_NATIVE>::apply");
+    return true;
+}
+template <typename T> size_t chol_ws(linalg_int lda, cholesky_decomposition_enum type)
+{
+    if (type != cholesky_decomposition_enum::LOWER_TRIANGULAR && type != cholesky_decomposition_enum::UPPER_TRIANGULAR)
+        throw std::invalid_argument("cholesky_decomposition<ROCM_NATIVE>: unknown triangle");
+    size_t bytes = 0;
+    check(ops<T>::chol_ws(lda, &bytes), "cholesky_decomposition<ROCM_NATIVE>::workspace_size");
+    return bytes;
+}
+
+template <> bool cholesky_decomposition<double, linalg_backend::ROCM_NATIVE>::apply(double* C, linalg_int lda,
+    cholesky_decomposition_enum type, void* workspace, size_t workspace_size, int* info, void* stream)
+{
+    return chol_apply<double>(C, lda, type, workspace, workspace_size, info, stream);
+}
+template <> bool cholesky_decomposition<float, linalg_backend::ROCM_NATIVE>::apply(float* C, linalg_int lda,
+    cholesky_decomposition_enum type, void* workspace, size_t workspace_size, int* info, void* stream)
+{
+    return chol_apply<float>(C, lda, type, workspace, workspace_size, info, stream);
+}
+template <> size_t cholesky_decomposition<double, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    cholesky_decomposition_enum type)
+{
+    return chol_ws<double>(lda, type);
+}
+template <> size_t cholesky_decomposition<float, linalg_backend::ROCM_NATIVE>::workspace_size(linalg_int lda,
+    cholesky_decomposition_enum type)
+{
+    return chol_ws<float>(lda, type);
 }
 
 namespace rocm_native

@@ -113,6 +113,22 @@ static void host_checks()
     EXPECT(throws<std::invalid_argument>([&] { DT::workspace_size(4, static_cast<linear_solver_type>(5)); }));
     EXPECT(throws<std::invalid_argument>([&] { IV::apply(nullptr, nullptr, 4, linear_solver_type::LU_LINEAR_SOLVER, ws, sizeof(ws)); }));
     EXPECT(throws<std::invalid_argument>([&] { IV::apply(m, nullptr, 4, static_cast<linear_solver_type>(5), ws, sizeof(ws)); }));
+    // cholesky_decomposition: device info required (the reference's GPU contract), triangle checked
+    using CH = linalg::cholesky_decomposition<double, linalg::linalg_backend::ROCM_NATIVE>;
+    int dinfo = 0;
+    EXPECT(CH::workspace_size(100, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR) > 0);
+    EXPECT(CH::workspace_size(100, linalg::cholesky_decomposition_enum::UPPER_TRIANGULAR) ==
+           CH::workspace_size(100, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { CH::apply(m, 4, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR, ws, sizeof(ws), nullptr); }));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { CH::apply(nullptr, 4, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR, ws, sizeof(ws), &dinfo); }));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { CH::apply(m, 0, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR, ws, sizeof(ws), &dinfo); }));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { CH::apply(m, 4, static_cast<linalg::cholesky_decomposition_enum>('X'), ws, sizeof(ws), &dinfo); }));
+    EXPECT(throws<std::invalid_argument>(
+        [&] { CH::apply(m, 4, linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR, ws, 1, &dinfo); }));
     // batched multiplication: the reference's checks (pointers, then sizes) before any launch
     EXPECT(throws<std::invalid_argument>(
         [&] { linalg::rocm_native::batched_matrix_multiplication(2, 2, static_cast<const double*>(nullptr), m, m); }));
@@ -315,6 +331,62 @@ static void gpu_checks()
         (void)hipFree(d_c);
         (void)hipFree(d_m);
         (void)hipFree(d_w);
+    }
+    {
+        // cholesky_decomposition<ROCM_NATIVE>: A = C C^T with C lower triangular, 5 on the diagonal and
+        // small positive entries below (the shape of the reference's known answer,
+        // TestCholeskyDecomposition.cxx:170-210), gives back C (LOWER) and C^T (UPPER) with device
+        // info 0; diag(4, -1) fails at order 2.
+        using CH = linalg::cholesky_decomposition<double, linalg::linalg_backend::ROCM_NATIVE>;
+        const long long nc = 70;
+        std::vector<double> c(nc * nc, 0.0), a(nc * nc, 0.0);
+        Normal u(77);
+        for (long long i = 0; i < nc; ++i)
+            for (long long j = 0; j <= i; ++j) c[i * nc + j] = (i == j) ? 5.0 : 0.5 * std::fabs(u()) / 3.0;
+        for (long long i = 0; i < nc; ++i)
+            for (long long j = 0; j < nc; ++j) {
+                double acc = 0.0;
+                for (long long p = 0; p < nc; ++p) acc += c[i * nc + p] * c[j * nc + p];
+                a[i * nc + j] = acc;
+            }
+        for (int up = 0; up < 2; ++up) {
+            const auto tri = up ? linalg::cholesky_decomposition_enum::UPPER_TRIANGULAR
+                                : linalg::cholesky_decomposition_enum::LOWER_TRIANGULAR;
+            const size_t wsb = CH::workspace_size(nc, tri);
+            double* d_c = nullptr;
+            int* d_info = nullptr;
+            void* d_w = nullptr;
+            HIP_OK(hipMalloc(&d_c, sizeof(double) * nc * nc));
+            HIP_OK(hipMalloc(&d_info, sizeof(int)));
+            HIP_OK(hipMalloc(&d_w, wsb));
+            HIP_OK(hipMemcpy(d_c, a.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
+            HIP_OK(hipMemset(d_info, 0x7f, sizeof(int)));
+            EXPECT(CH::apply(d_c, nc, tri, d_w, wsb, d_info));
+            HIP_OK(hipDeviceSynchronize());
+            std::vector<double> f(nc * nc);
+            int hinfo = -1;
+            HIP_OK(hipMemcpy(f.data(), d_c, sizeof(double) * nc * nc, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost));
+            double err = 0.0;
+            for (long long i = 0; i < nc; ++i)
+                for (long long j = 0; j < nc; ++j) {
+                    const double want = up ? c[j * nc + i] : c[i * nc + j];
+                    err = std::fmax(err, std::fabs(f[i * nc + j] - want));
+                }
+            std::printf("cholesky<ROCM_NATIVE> %s n=%lld: info %d, max|factor - C| %.2e\n", up ? "upper" : "lower", nc,
+                hinfo, err);
+            EXPECT(hinfo == 0);
+            EXPECT(err <= 200 * 2.220446049250313e-16 * 5.0);
+            const double bad[4] = {4.0, 0.0, 0.0, -1.0};
+            HIP_OK(hipMemcpy(d_c, bad, sizeof(bad), hipMemcpyHostToDevice));
+            EXPECT(CH::apply(d_c, 2, tri, d_w, wsb, d_info));
+            HIP_OK(hipDeviceSynchronize());
+            HIP_OK(hipMemcpy(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost));
+            EXPECT(hinfo == 2);
+            (void)hipFree(d_c);
+            (void)hipFree(d_info);
+            (void)hipFree(d_w);
+        }
     }
 }
 

@@ -53,6 +53,8 @@ def test_cxx_specialisations_exported():
         assert f"linalg::lu_decomposition<{t}, (linalg::linalg_backend)6>::workspace_size" in text
         assert f"linalg::linear_solver<{t}, (linalg::linalg_backend)6>::apply" in text
         assert f"linalg::linear_solver<{t}, (linalg::linalg_backend)6>::workspace_size" in text
+        assert f"linalg::cholesky_decomposition<{t}, (linalg::linalg_backend)6>::apply" in text
+        assert f"linalg::cholesky_decomposition<{t}, (linalg::linalg_backend)6>::workspace_size" in text
     assert "linalg::rocm_native::gesv_mixed" in text
 
 

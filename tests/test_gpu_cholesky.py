@@ -191,3 +191,24 @@ def test_cholesky_inversion(dim, dtype, tol, dev):
     lo, _ = O.cholesky(a)
     inv2 = _inv(lo, dev, 4).astype(np.float64)
     assert np.abs(inv2 @ a.astype(np.float64) - eye).max() <= tol
+
+
+@pytest.mark.parametrize("n,fail_row", [(256, 10), (300, 150), (1000, 700)])
+def test_cholesky_failure_leaves_reference_partial_factor(n, fail_row, dev):
+    """A leading minor that fails at row fail_row (block 0, a middle block, a late block): info
+    names it, and the lower triangle is what the reference leaves when it stops there
+    (cholesky_decomposition_native.cpp:245-260): the completed rows factored, the failing row's
+    multipliers, the failing diagonal entry and every later row as the trailing updates of the
+    completed blocks left them. The trailing update is skipped after the failing block."""
+    rng = np.random.default_rng(n + fail_row)
+    g = rng.standard_normal((n, n))
+    a = g @ g.T / n + np.eye(n)
+    a[fail_row, fail_row] = -1.0
+    r, info, ok = _chol_gpu(a, dev)
+    ref, info_ref = O.cholesky(a)
+    lower = np.tril_indices(n)
+    d = O.rel_diff(r[lower], ref[lower])
+    print(f"cholesky failure n={n} row={fail_row}: info {info} (oracle {info_ref}), lower-triangle rel diff {d:.2e}",
+          flush=True)
+    assert not ok and info == info_ref == fail_row + 1
+    assert d <= 1e-13
