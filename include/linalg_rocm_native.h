@@ -38,7 +38,8 @@ enum lrn_status {
     LRN_E_HIP = 1,          /* a HIP runtime call failed */
     LRN_E_TIMEOUT = 2,      /* a bounded inter-workgroup wait expired (result invalid) */
     LRN_E_UNSUPPORTED = 3,  /* size outside what this build supports */
-    LRN_E_INTERNAL = 4
+    LRN_E_INTERNAL = 4,
+    LRN_E_SCRATCH = 5       /* order too large for the allocation-free entry point's scratch */
 };
 
 /* linear_solver_type values, identical to the reference enum
@@ -83,6 +84,20 @@ LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double
     void* workspace, size_t workspace_size, void* stream);
 LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* x, int type,
     void* workspace, size_t workspace_size, void* stream);
+
+/* Allocation-free variant of lrn_linear_solver_f64 (no counterpart in the reference): for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, semantics and results (bitwise) as
+ * lrn_linear_solver_f64, minus the workspace: the work areas live in a 4 MiB device array of
+ * the library (one per GPU, held until the process ends), and the factorisation runs on
+ * `stream` alone. Orders whose workspace does not fit return LRN_E_SCRATCH
+ * (lrn_linear_solver_noalloc_max_order gives the largest order per solver type). The call
+ * allocates nothing. Calls that overlap on one GPU share that GPU's array and RACE: keep them
+ * on one stream or otherwise ordered. Argument errors: -1 m, -2 pivot (LU modes), -3 lda,
+ * -4 x, -5 type. ASYNCHRONOUS on stream. */
+LRN_API int lrn_linear_solver_noalloc_f64(double* m, int64_t* pivot, int64_t lda, double* x, int type,
+    void* stream);
+LRN_API int lrn_linear_solver_noalloc_max_order(int type, int64_t* order);
 
 /* ---- triangular_solver<T, ROCM_NATIVE> ------------------------------------------------
  * Replaces triangular_solver<T, B>::apply / workspace_size

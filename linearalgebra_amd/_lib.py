@@ -27,6 +27,8 @@ SIGNATURES = {
     "lrn_linear_solver_workspace_size_f32": [_i64, _int, ctypes.POINTER(_sz)],
     "lrn_linear_solver_f64": [_vp, _vp, _i64, _vp, _int, _vp, _sz, _vp],
     "lrn_linear_solver_f32": [_vp, _vp, _i64, _vp, _int, _vp, _sz, _vp],
+    "lrn_linear_solver_noalloc_f64": [_vp, _vp, _i64, _vp, _int, _vp],
+    "lrn_linear_solver_noalloc_max_order": [_int, ctypes.POINTER(_i64)],
     "lrn_triangular_solver_workspace_size_f64": [_i64, _i64, ctypes.POINTER(_sz)],
     "lrn_triangular_solver_workspace_size_f32": [_i64, _i64, ctypes.POINTER(_sz)],
     "lrn_triangular_solver_f64": [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp],

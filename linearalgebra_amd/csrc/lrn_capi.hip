@@ -1970,6 +1970,44 @@ int batched_gesv_impl(T* a, const int64_t* a_off, int64_t* piv, const int64_t* p
 }  // namespace
 }  // namespace lrn
 
+// ------------------------------------------------------------------- allocation-free solve
+// Scratch of lrn_linear_solver_noalloc_f64: a file-scope device array that takes the place of
+// the caller's workspace (each GPU holds its own copy, resident until the process ends). In
+// host code the array's name is only a stand-in for the symbol, never a device address: the
+// address comes from hipGetSymbolAddress, once per device with that device current, and only
+// that address is handed to kernels.
+constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace lrn {
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::atomic<void*> g_scratch_ptr[kNoAllocMaxDevices];
+
+// Device address of this device's scratch (looked up on the device's first call).
+int noalloc_scratch(void** out)
+{
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return LRN_E_UNSUPPORTED;
+    void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (p == nullptr) {
+        static std::mutex mu;
+        std::lock_guard<std::mutex> g(mu);
+        p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+        if (p == nullptr) {
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return LRN_E_HIP;
+            g_scratch_ptr[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return LRN_OK;
+}
+
+}  // namespace
+}  // namespace lrn
+
 using namespace lrn;
 
 extern "C" {
@@ -1980,6 +2018,7 @@ LRN_API const char* lrn_status_string(int status)
     case LRN_OK: return "ok";
     case LRN_E_HIP: return "HIP runtime call failed";
     case LRN_E_TIMEOUT: return "bounded inter-workgroup wait expired";
+    case LRN_E_SCRATCH: return "order too large for the allocation-free entry point's 4 MiB device scratch";
     case LRN_E_UNSUPPORTED: return "unsupported size or mode";
     case LRN_E_INTERNAL: return "internal error";
     default: return status < 0 ? "invalid argument" : "unknown status";
@@ -2057,6 +2096,36 @@ LRN_API int lrn_linear_solver_f64(double* m, int64_t* pivot, int64_t lda, double
 {
     return solver_impl<double>(m, pivot, lda, x, 1, 1, type, workspace, workspace_size, stream);
 }
+LRN_API int lrn_linear_solver_noalloc_f64(double* m, int64_t* pivot, int64_t lda, double* x, int type, void* stream)
+{
+    if (m == nullptr) return -1;
+    if (lda <= 0) return -3;
+    if (x == nullptr) return -4;
+    if (type < LRN_LU_LINEAR_SOLVER || type > LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return -5;
+    if (pivot == nullptr && is_lu_type(type)) return -2;
+    if (lda > kMaxOrder) return LRN_E_UNSUPPORTED;
+    if (solver_ws_bytes<double>(lda, 1, type) > kNoAllocScratchBytes) return LRN_E_SCRATCH;
+    void* ws = nullptr;
+    LRN_TRY(noalloc_scratch(&ws));
+    return solver_impl<double>(m, pivot, lda, x, 1, 1, type, ws, kNoAllocScratchBytes, stream, true);
+}
+
+LRN_API int lrn_linear_solver_noalloc_max_order(int type, int64_t* order)
+{
+    if (type < LRN_LU_LINEAR_SOLVER || type > LRN_CHOLESKY_UPFRONT_LINEAR_SOLVER) return -1;
+    if (order == nullptr) return -2;
+    long long lo = 0, hi = kMaxOrder;  // largest order whose workspace fits the scratch
+    while (lo < hi) {
+        const long long mid = (lo + hi + 1) / 2;
+        if (solver_ws_bytes<double>(mid, 1, type) <= kNoAllocScratchBytes)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    *order = lo;
+    return LRN_OK;
+}
+
 LRN_API int lrn_linear_solver_f32(float* m, int64_t* pivot, int64_t lda, float* x, int type, void* workspace,
     size_t workspace_size, void* stream)
 {
