@@ -2,8 +2,9 @@
 
     python -m linearalgebra_amd.build
 
-compiles ``csrc/lrn_capi.hip`` (kernels + host driver + C ABI) into
-``liblinalg_rocm_native.so`` next to this file, and the C++ backend specialisation
+compiles the translation units of ``csrc/`` (``lrn_capi.hip``: kernels + host driver + C ABI;
+``lrn_batched.hip``: the batched solver's kernels) in parallel and links them into
+``liblinalg_rocm_native.so`` next to this file, then the C++ backend specialisation
 (``csrc/rocm_native_backend.cpp``) into ``liblinalg_rocm_native_cxx.so`` linked against it.
 """
 from __future__ import annotations
@@ -21,10 +22,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no packed FP32 instructions are generated (they are avoided on purpose).
 # -ffp-contract=off: fused multiply-adds only where the source asks for fma()/MFMA.
 HIP_FLAGS = [
-    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-fvisibility=hidden", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
+# translation units of the library (each launches only the kernels it defines)
+HIP_UNITS = ["lrn_capi.hip", "lrn_batched.hip"]
+OBJ_DIR = os.path.join(HERE, "_obj")
 
 
 def _run(cmd):
@@ -42,9 +46,18 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
 
 
 def build(verbose: bool = False) -> str:
-    """The HIP library, then the C++ ROCM_NATIVE specialisations linked against it."""
+    """The HIP library (its translation units compiled in parallel, then linked), then the C++
+    ROCM_NATIVE specialisations linked against it."""
+    from concurrent.futures import ThreadPoolExecutor
+
     inc = ["-I", os.path.join(ROOT, "include")]
-    out = _run([HIPCC, *HIP_FLAGS, *inc, "-o", LIB, os.path.join(CSRC, "lrn_capi.hip")])
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, u.replace(".hip", ".o")) for u in HIP_UNITS]
+    with ThreadPoolExecutor(max_workers=len(HIP_UNITS)) as ex:
+        outs = list(ex.map(lambda uo: _run([HIPCC, *HIP_FLAGS, *inc, "-c", "-o", uo[1], os.path.join(CSRC, uo[0])]),
+                           zip(HIP_UNITS, objs)))
+    out = "".join(outs)
+    out += _run([HIPCC, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", LIB, *objs])
     out += _run(["g++", *CXX_FLAGS, "-shared", *inc, "-o", CXX_LIB, os.path.join(CSRC, "rocm_native_backend.cpp"),
                  "-L", HERE, "-llinalg_rocm_native", "-Wl,-rpath,$ORIGIN"])
     if verbose and out.strip():

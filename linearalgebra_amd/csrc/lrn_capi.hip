@@ -35,7 +35,7 @@
 #include "lrn_gemm_bf16.hpp"
 #include "lrn_mixed.hpp"
 #include "lrn_trace.hpp"
-#include "lrn_batched.hpp"
+#include "lrn_batched_launch.hpp"
 #include "lrn_calu.hpp"
 #include "lrn_inverse.hpp"
 #include "lrn_cholesky.hpp"
@@ -1946,25 +1946,9 @@ int batched_gesv_impl(T* a, const int64_t* a_off, int64_t* piv, const int64_t* p
     if (count == 0) return LRN_OK;
     Arena ar(ws, ws_size);
     const BatchLayout L = carve_batch(ar, count);
-    hipLaunchKernelGGL(batch_order_kernel, dim3(1), dim3(1024), 0, s, n, nrhs, static_cast<int>(count), L.order, L.ctr,
-        info);
-    LRN_HIP(hipGetLastError());
-    constexpr int NB = 16;
-    auto kern = batched_gesv_kernel<T, NB>;
-    int per_cu = 0;
-    LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(kern), 0, kBatThreads, &per_cu));
-    int dev = 0;
-    LRN_HIP(hipGetDevice(&dev));
-    long long grid = static_cast<long long>(std::max(per_cu, 1)) * device_props(dev).cus;
-    if (g_batch_grid.load() > 0) grid = g_batch_grid.load();
-    grid = std::min<long long>(grid, count);
-    TraceScope ts(kTrBatched, 0.0, s);
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(kBatThreads), 0, s, a,
-        reinterpret_cast<const long long*>(a_off), reinterpret_cast<long long*>(piv),
+    return batched_solve_launch<T>(a, reinterpret_cast<const long long*>(a_off), reinterpret_cast<long long*>(piv),
         reinterpret_cast<const long long*>(piv_off), b, reinterpret_cast<const long long*>(b_off), n, nrhs, info,
-        L.order, L.ctr, static_cast<int>(count));
-    LRN_HIP(hipGetLastError());
-    return LRN_OK;
+        L.order, L.ctr, static_cast<int>(count), g_batch_grid.load(), s);
 }
 
 }  // namespace

@@ -2,11 +2,18 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <stdint.h>
 
 namespace lrn {
 
 constexpr int kWave = 64;
+
+// Smallest normal number of the type: the reference divides instead of multiplying by the
+// reciprocal below it (src/matrix_operation/cpu/native/lu_decomposition_native.cpp:212-228).
+template <typename T> struct TypeMin;
+template <> struct TypeMin<double> { static constexpr double v = DBL_MIN; };
+template <> struct TypeMin<float> { static constexpr float v = FLT_MIN; };
 
 // Pivot-search order: larger magnitude wins; equal magnitudes go to the lower row index,
 // which is the "first maximum wins" rule of a strict '>' scan in increasing row order

@@ -35,9 +35,6 @@ template <typename T> struct PanelWs {
     void* dchunks;
 };
 
-template <typename T> struct TypeMin;
-template <> struct TypeMin<double> { static constexpr double v = DBL_MIN; };
-template <> struct TypeMin<float> { static constexpr float v = FLT_MIN; };
 
 // Net row movement of one panel's interchanges. The swaps (i, piv[i]-1), i = k..k+w-1, are
 // replayed on position labels by one wave: lane l tracks position k+l, and positions at or

@@ -1,0 +1,93 @@
+"""Launches one kernel kind of the library repeatedly on resident inputs, for rocprofv3 runs
+(kernel traces and counters need a program that does nothing else):
+
+    python -m linearalgebra_amd.profile_kernels gemm_f64 --m 16128 --n 16128 --k 256 --reps 5
+    python -m linearalgebra_amd.profile_kernels gemm_f32 --m 16128 --n 16128 --k 256 --reps 5
+    python -m linearalgebra_amd.profile_kernels copy --m 16384 --n 16384 --reps 5
+    python -m linearalgebra_amd.profile_kernels batched_f64 --count 10000 --reps 5
+
+gemm_*: C[m x n] -= A[m x k] B[k x n] (row-major, the trailing-update shape of the blocked
+LU at order n + k with a k-wide outer block) through lrn_debug_gemm_sub. copy: a device copy
+of an m x n FP64 matrix (the calibration of the HBM byte counters). batched_*: the batched
+solve of `count` systems, n ~ U[8, 256], nrhs ~ U[1, 8], iid N(0, 1) (the inputs are restored
+by a device copy before each launch). Every launch goes to the current stream; the program
+synchronises at the end and prints one line per phase.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import sys
+import time
+
+
+def main() -> None:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("kind", choices=["gemm_f64", "gemm_f32", "gemm_bf16x1", "gemm_bf16x2", "gemm_bf16x3", "copy",
+                                    "batched_f64", "batched_f32"])
+    p.add_argument("--m", type=int, default=16128)
+    p.add_argument("--n", type=int, default=16128)
+    p.add_argument("--k", type=int, default=256)
+    p.add_argument("--count", type=int, default=10000)
+    p.add_argument("--reps", type=int, default=5)
+    p.add_argument("--seed", type=int, default=2024)
+    a = p.parse_args()
+
+    import torch
+
+    from . import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(a.seed)
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    t0 = time.perf_counter()
+    if a.kind.startswith("gemm"):
+        dtype = torch.float64 if a.kind == "gemm_f64" else torch.float32
+        code = {"gemm_f64": 0, "gemm_f32": 1, "gemm_bf16x1": 2, "gemm_bf16x2": 3, "gemm_bf16x3": 4}[a.kind]
+        A = torch.randn(a.m, a.k, dtype=dtype, device=dev, generator=g)
+        B = torch.randn(a.k, a.n, dtype=dtype, device=dev, generator=g)
+        C = torch.randn(a.m, a.n, dtype=dtype, device=dev, generator=g)
+        torch.cuda.synchronize()
+        print(f"{a.kind}: {a.m} x {a.n} x {a.k}, {a.reps} launches", flush=True)
+        for _ in range(a.reps):
+            rc = lib.lrn_debug_gemm_sub(code, vp(A), a.k, vp(B), a.n, vp(C), a.n, a.m, a.n, a.k, s)
+            if rc != 0:
+                sys.exit(f"lrn_debug_gemm_sub returned {rc}")
+    elif a.kind == "copy":
+        src = torch.randn(a.m, a.n, dtype=torch.float64, device=dev, generator=g)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        print(f"copy: {a.m} x {a.n} FP64 ({src.numel() * 8} bytes), {a.reps} launches", flush=True)
+        for _ in range(a.reps):
+            dst.copy_(src)
+    else:
+        import numpy as np
+
+        from .batched import BatchedSystems
+
+        dtype = torch.float64 if a.kind == "batched_f64" else torch.float32
+        rng = np.random.Generator(np.random.PCG64(a.seed))
+        n = rng.integers(8, 257, size=a.count)
+        r = rng.integers(1, 9, size=a.count)
+        sy = BatchedSystems(n, r, dtype=dtype, device=dev)
+        sy.a.copy_(torch.randn(sy.a.numel(), dtype=dtype, device=dev, generator=g))
+        sy.b.copy_(torch.randn(sy.b.numel(), dtype=dtype, device=dev, generator=g))
+        a0, b0 = sy.a.clone(), sy.b.clone()
+        torch.cuda.synchronize()
+        print(f"{a.kind}: {a.count} systems, {a.reps} launches", flush=True)
+        for _ in range(a.reps):
+            sy.a.copy_(a0)
+            sy.b.copy_(b0)
+            sy.solve()
+        bad = int((sy.info != 0).sum())
+        if bad:
+            sys.exit(f"{bad} systems reported a nonzero info")
+    torch.cuda.synchronize()
+    print(f"done in {time.perf_counter() - t0:.2f} s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

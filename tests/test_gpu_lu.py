@@ -37,6 +37,17 @@ def _gpu_solve(a, b, dev, mode=1, lu=None, piv=None):
     return m.cpu().numpy(), p.cpu().numpy(), x.cpu().numpy()
 
 
+def _kappa_inf(a):
+    """LAPACK's infinity-norm condition estimate (dgecon on scipy's getrf factors)."""
+    import scipy.linalg as sl
+    from scipy.linalg import lapack
+    lu, _ = sl.lu_factor(a, check_finite=False)
+    anorm = np.abs(a).sum(axis=1).max()
+    rcond, info = lapack.dgecon(lu, anorm, norm="I")
+    assert info == 0
+    return 1.0 / rcond if rcond > 0 else np.inf
+
+
 SMALL = [1, 2, 3, 11, 17, 63, 64, 65, 127, 128, 129, 200, 255, 256, 257, 300, 511, 700]
 
 
@@ -126,10 +137,15 @@ def test_solver_lu_mode_matches_oracle(n, nrhs, dev):
     lu_o, piv_o, x_o, info_o = O.gesv(a, b_in, threads=16)
     fe = O.forward_error(x_g, x_o)
     be = O.backward_error(a, x_g, b_in)
-    print(f"n={n} nrhs={nrhs} fwd_vs_oracle={fe:.3e} berr={be:.3e}", flush=True)
+    be_o = O.backward_error(a, x_o, b_in)
+    # two backward-stable solutions differ by at most about kappa(A) (berr_1 + berr_2): the bar
+    # is 1e-12 where the conditioning allows it, that bound (with a factor 10) beyond
+    kappa = _kappa_inf(a)
+    bar = max(1e-12, 10.0 * kappa * (be + be_o))
+    print(f"n={n} nrhs={nrhs} fwd_vs_oracle={fe:.3e} berr={be:.3e} kappa_inf~{kappa:.2e} bar={bar:.2e}", flush=True)
     assert np.array_equal(piv_g, piv_o)
     assert be <= 4 * n * np.finfo(np.float64).eps
-    assert fe <= 1e-10
+    assert fe <= bar
 
 
 def test_solver_upfront_reuses_factors(dev):
@@ -265,3 +281,4 @@ def test_block_update_kernels_match_oracle(kind, dev):
         lib.lrn_set_block_update(0)
         lib.lrn_set_lookahead(0)
         autotune.apply(lib)
+
