@@ -17,7 +17,8 @@ every launch of the traced kernel kinds) runs during the timed steps; `roofline`
 dominant matrix-core kernel of the measured path (algorithmic flops 2*M*N*K per launch,
 summed over launches, divided by the summed launch durations). `phases` gives every kernel
 kind from one extra traced step. `cpu_baseline` is the CPU restatement of the reference's
-native algorithm timed on this host on a smaller order and scaled by n^3.
+native algorithm (single thread) and LAPACK (1 thread and all cores) timed on this host at
+n = 1024, 2048, 4096 and on the CPU configuration (n = 1024, nrhs = 16), fitted by n^3.
 
 Configuration grids (skipped with --no-configs): `plain_configs` times the plain FP64 LU +
 solve at n = 4096, 8192, 16384 with nrhs 1 and 64 on A iid N(0,1) and on A + n*I, each with
@@ -27,7 +28,7 @@ forward difference from the plain FP64 solution (--mixed-extra-n 32768 adds that
 `triangular_solver` times op(A) X = B at the headline order for nrhs 1 and 64 and four flag
 combinations (HBM rate of the algorithmic bytes, residual).
 
-Usage: python bench.py [--steps K] [--warmup W] [--n N] [--kappa K] [--cpu-n N] [--no-cpu]
+Usage: python bench.py [--steps K] [--warmup W] [--n N] [--kappa K] [--cpu-sizes N,..] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -66,7 +67,7 @@ def parse():
     p.add_argument("--kappa", type=float, default=1e6)
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--restart", type=int, default=30)
-    p.add_argument("--cpu-n", type=int, default=2048, help="order of the CPU baseline sample")
+    p.add_argument("--cpu-sizes", default="1024,2048,4096", help="orders of the CPU baseline samples")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--matrix", choices=["conditioned", "gaussian"], default="conditioned",
                    help="conditioned: U diag(s) V^T with the given kappa (the benchmark input); gaussian: iid "
@@ -573,35 +574,101 @@ def batched_mixed_bench(torch, count, seed, dev):
 
 # ----------------------------------------------------------------------------- CPU baseline
 
-def cpu_baseline(n_cpu, n_target, seed):
-    """The CPU restatement of the reference's native LU + lu_solve (single thread, as the
-    reference is single-threaded), timed at n_cpu and scaled to n_target by n^3."""
+def _host_description():
+    import platform
+    cores = len(os.sched_getaffinity(0))
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def _fit_cubic(ns, ts, n_target):
+    """Least-squares c in t = c n^3 over the samples, and the free log-log exponent."""
+    import numpy as np
+    ns = np.asarray(ns, dtype=np.float64)
+    ts = np.asarray(ts, dtype=np.float64)
+    c = float((ts * ns ** 3).sum() / (ns ** 6).sum())
+    slope = float(np.polyfit(np.log(ns), np.log(ts), 1)[0]) if len(ns) > 1 else 3.0
+    return c, slope, c * float(n_target) ** 3
+
+
+def cpu_baseline(sizes, n_target, seed):
+    """CPU baseline on this host, in the same run as the GPU measurements.
+
+    native: the CPU restatement of the reference's native LU + lu_solve (oracle/lu_oracle.c,
+            64-wide blocked right-looking LU with partial pivoting, scalar i-k-j GEMM, every
+            product rounded), single thread as the reference is single-threaded, timed at each
+            order in `sizes` (nrhs = 1) and on the CPU configuration itself (n = 1024, nrhs = 16);
+    lapack: LAPACK dgetrf + dgetrs through scipy (row-major A passed as the column-major
+            transpose, as LAPACKE_ROW_MAJOR does) at 1 thread (the reference's published
+            methodology) and on every host core, at the same orders.
+    Each is extrapolated to n_target by a least-squares fit t = c n^3 (the fit and the free
+    log-log exponent are reported). `value` is the native path's extrapolated solves/s."""
     sys.path.insert(0, HERE)
     import numpy as np
     from oracle import oracle as O
 
     O.build()
-    a = O.gaussian(n_cpu, seed=seed)
-    b = O.gaussian(n_cpu, 1, seed=seed + 1)[:, 0]
+    cores, model = _host_description()
+    native = []
+    for n in sizes:
+        a = O.gaussian(n, seed=seed + n)
+        b = O.gaussian(n, 1, seed=seed + n + 1)[:, 0]
+        t0 = time.perf_counter()
+        _, _, x, info = O.gesv(a, b, threads=1)
+        dt = time.perf_counter() - t0
+        assert info == 0 and np.isfinite(x).all()
+        native.append({"n": n, "nrhs": 1, "seconds": round(dt, 4), "gflops": round(2.0 * n ** 3 / 3.0 / dt / 1e9, 3)})
+        log(f"cpu native n={n}: {dt:.3f} s")
+    # the CPU configuration itself: n = 1024, 16 right-hand sides
+    a = O.gaussian(1024, seed=seed)
+    b = O.gaussian(1024, 16, seed=seed + 1)
     t0 = time.perf_counter()
     _, _, x, info = O.gesv(a, b, threads=1)
-    dt = time.perf_counter() - t0
-    assert info == 0 and np.isfinite(x).all()
-    scale = (n_target / n_cpu) ** 3
-    res = {"value": round(1.0 / (dt * scale), 6), "unit": "solves/s", "cores": 1, "kind": "port",
-           "sample": f"oracle getrf+getrs (FP64, partial pivoting, N(0,1)) at n={n_cpu}, nrhs=1, 1 thread: "
-                     f"{dt:.2f} s; scaled by (n/{n_cpu})^3 to n={n_target}"}
-    # vendor LAPACK beside it (scipy's OpenBLAS), all host cores
+    dt0 = time.perf_counter() - t0
+    assert info == 0
+    config0 = {"n": 1024, "nrhs": 16, "seconds": round(dt0, 4), "solves_per_s": round(1.0 / dt0, 3),
+               "gflops": round((2.0 * 1024 ** 3 / 3.0 + 2.0 * 1024 ** 2 * 16) / dt0 / 1e9, 3)}
+    c, slope, t_target = _fit_cubic([r["n"] for r in native], [r["seconds"] for r in native], n_target)
+    res = {"value": round(1.0 / t_target, 6), "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": f"oracle getrf+getrs (FP64, partial pivoting, N(0,1), nrhs=1), 1 thread, at n = "
+                     f"{', '.join(str(n) for n in sizes)}; extrapolated to n={n_target} by t = c n^3 "
+                     f"(c = {c:.3e} s, free log-log exponent {slope:.2f})",
+           "host": {"cores": cores, "model": model},
+           "native": native, "config0": config0,
+           "extrapolation": {"form": "t = c n^3, least squares", "c_seconds": c, "loglog_exponent": round(slope, 3),
+                             "seconds_at_target": round(t_target, 2)}}
     try:
+        import scipy
         import scipy.linalg as sl
-        cores = len(os.sched_getaffinity(0))
-        t0 = time.perf_counter()
-        lu, piv = sl.lu_factor(a, check_finite=False)
-        sl.lu_solve((lu, piv), b, check_finite=False)
-        dl = time.perf_counter() - t0
-        res["lapack"] = {"value": round(1.0 / (dl * scale), 6), "unit": "solves/s", "cores": cores,
-                         "sample": f"scipy lu_factor+lu_solve (OpenBLAS dgetrf/dgetrs) at n={n_cpu}, "
-                                   f"{cores} host cores: {dl:.3f} s; scaled by n^3 to n={n_target}"}
+        from threadpoolctl import threadpool_info, threadpool_limits
+        libs = [f"{i.get('internal_api')} {i.get('version')}" for i in threadpool_info()]
+        out = {}
+        for label, nthreads in (("1_thread", 1), ("all_cores", cores)):
+            rows = []
+            with threadpool_limits(limits=nthreads):
+                for n in sizes:
+                    a = O.gaussian(n, seed=seed + n)
+                    b = O.gaussian(n, 1, seed=seed + n + 1)[:, 0]
+                    at = np.asfortranarray(a)  # the row-major matrix handed to LAPACK column-major
+                    sl.lu_factor(at, check_finite=False)  # warm-up
+                    t0 = time.perf_counter()
+                    lu, piv = sl.lu_factor(at, check_finite=False)
+                    sl.lu_solve((lu, piv), b, check_finite=False)
+                    dt = time.perf_counter() - t0
+                    rows.append({"n": n, "seconds": round(dt, 5), "gflops": round(2.0 * n ** 3 / 3.0 / dt / 1e9, 2)})
+            cl, sll, tl = _fit_cubic([r["n"] for r in rows], [r["seconds"] for r in rows], n_target)
+            out[label] = {"threads": nthreads, "samples": rows, "value": round(1.0 / tl, 5), "unit": "solves/s",
+                          "extrapolation": {"c_seconds": cl, "loglog_exponent": round(sll, 3)}}
+            log(f"cpu lapack {label}: " + ", ".join(f"n={r['n']} {r['seconds']:.3f} s" for r in rows))
+        res["lapack"] = {"library": f"scipy {scipy.__version__} LAPACK ({'; '.join(sorted(set(libs)))})", **out}
     except Exception as e:  # noqa: BLE001 - optional extra
         res["lapack"] = f"unavailable: {e}"
     return res
@@ -742,8 +809,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline at n={args.cpu_n}")
-        cpu = cpu_baseline(args.cpu_n, n, args.seed)
+        log(f"CPU baseline at n={args.cpu_sizes}")
+        cpu = cpu_baseline([int(v) for v in args.cpu_sizes.split(",") if v], n, args.seed)
 
     last = stats[-1]
     value = steps / t_mixed
