@@ -523,8 +523,9 @@ def batched_bench(torch, count, replays, seed, dev):
 def batched_mixed_bench(torch, count, seed, dev):
     """The batched mixed-precision workload: `count` FP64 systems, n log-uniform on
     [256, 4096], each U diag(s) V^T with kappa log-uniform on [1e1, 1e8]; trailing update
-    with two bf16 planes where kappa <= 1e4, FP32 matrix instructions otherwise; FP64
-    refinement to backward error <= 1e-14. Inputs generated on the GPU (not timed)."""
+    with three block-scaled FP8 e4m3 planes where kappa <= 1e4, FP32 matrix instructions
+    otherwise; FP64 refinement to backward error <= 1e-14. Inputs generated on the GPU (not
+    timed)."""
     import numpy as np
     from linearalgebra_amd import mixed as lam
 
@@ -539,7 +540,7 @@ def batched_mixed_bench(torch, count, seed, dev):
     a = torch.cat(a_parts)
     b = torch.cat(b_parts)
     del a_parts, b_parts
-    modes = np.where(kappa <= 1e4, 1, 3).astype(np.int32)
+    modes = np.where(kappa <= 1e4, 6, 3).astype(np.int32)
     lam.gesv_mixed_batched(a, n[:1], b, modes=modes[:1])  # warm-up (kernel set-up)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -558,7 +559,7 @@ def batched_mixed_bench(torch, count, seed, dev):
     paths = [s["path_name"] for s in stats]
     log(f"batched mixed: {count} systems in {dt:.3f} s")
     return {"workload": f"{count} FP64 systems, n log-uniform [256,4096], kappa log-uniform [1e1,1e8], "
-                        "U diag(s) V^T; two bf16 planes where kappa <= 1e4, FP32 MFMA otherwise; one call",
+                        "U diag(s) V^T; three block-scaled FP8 e4m3 planes where kappa <= 1e4, FP32 MFMA otherwise; one call",
             "systems": count, "seconds": round(dt, 4), "solves_per_s": round(count / dt, 3),
             "fp64_equivalent_tflops": round(flops / dt / 1e12, 3),
             "max_backward_error": max(berrs), "paths": {p: paths.count(p) for p in sorted(set(paths))},

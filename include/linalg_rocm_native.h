@@ -248,7 +248,10 @@ typedef struct lrn_mixed_stats {
  * cores with FP32 accumulation. mode 0 = one plane (bf16 operands), 1 = two planes (three
  * products, ~2^-16 relative), 2 = three planes (six products, FP32-level factors),
  * 3 = FP32 matrix instructions (the default). Modes 2 and 3 give FP32-level factors, which
- * LU-IR needs at kappa ~ 1e6; modes 0 and 1 rely on GMRES-IR there. */
+ * LU-IR needs at kappa ~ 1e6; modes 0 and 1 rely on GMRES-IR there. Modes 4, 5, 6 = one, two,
+ * three planes of FP8 e4m3 with one power-of-two scale per 32 elements along K of each row of
+ * L21 / column of U12 (1, 3, 6 products on the block-scaled FP8 matrix cores, FP32
+ * accumulation; about 4, 8, 12 significant bits per element): for well-conditioned systems. */
 LRN_API int lrn_set_mixed_update(int mode);
 /* Panel of the FP32 factorisation inside the mixed solver (process-wide, no reference
  * counterpart): 1 = tournament pivoting (the pivot rows of each 32-column panel are chosen
@@ -262,8 +265,9 @@ LRN_API int lrn_set_mixed_panel(int tournament);
  * each solving its rows in 64-row sub-blocks. */
 LRN_API int lrn_set_refine_solver(int chain);
 /* Kernel of the 32-column cooperative panel (process-wide, no reference counterpart; same
- * pivots and factors either way): 0 = panel rows held in registers, two workgroup barriers
- * per column (the default); 1 = panel rows in LDS with deferred updates. */
+ * pivots and factors either way): 0 = panel rows held in registers, one row per thread of
+ * 256-thread workgroups, two exchanges and three workgroup barriers per column (the default);
+ * 1 = panel rows in LDS with deferred updates. */
 LRN_API int lrn_set_panel_kernel(int kind);
 /* The refinement's preconditioner on its own (no reference counterpart): b <- U^-1 L^-1 P b
  * with FP32 factors and pivots (as from lrn_getrf_tournament_f32 or lrn_getrf_async_f32)
@@ -329,10 +333,17 @@ LRN_API int lrn_set_block_update(int kind);
  * error, then GMRES-IR from the best iterate (the default); 1 = GMRES-IR right after the
  * first LU-IR step. Process-wide. */
 LRN_API int lrn_set_refinement(int mode);
+/* Many-right-hand-side solves (LU and Cholesky solver modes, triangular_solver): solves with at
+ * least min_rhs (and at least 2) right-hand sides use the blocked form (64 x 64 diagonal blocks
+ * inverted, then one matrix-core update launch per 64-row strip); fewer, or min_rhs = 0, the
+ * chained substitution kernels. Default 2. Process-wide. */
+LRN_API int lrn_set_multi_rhs_solver(int min_rhs);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,
- * 2, 3, 4 = f32 data split into 1, 2, 3 bf16 planes, FP32 accumulation (gemm_sub only).
+ * 2, 3, 4 = f32 data split into 1, 2, 3 bf16 planes, FP32 accumulation; 5, 6, 7 = f32 data
+ * split into 1, 2, 3 block-scaled FP8 e4m3 planes (scale per 32 K elements), FP32
+ * accumulation (gemm_sub only).
  * gemm_sub: C[M x N] -= A[M x K] * B[K x N] (row-major, leading dims lda/ldb/ldc).
  * trsm: rows k..k+w-1, columns [c0, c1) of the lda x lda matrix a <- L11^-1 a, with L11 the
  *       unit-lower w x w block at (k, k).
@@ -363,7 +374,11 @@ enum lrn_trace_kind {
     LRN_TRACE_TRSM = 7,            /* U12 triangular solve (bytes) */
     LRN_TRACE_TRSV = 8,            /* solve substitutions (bytes) */
     LRN_TRACE_GEMM_BF16X6 = 9,     /* three-plane bf16 trailing GEMM (FP32-equivalent flops 2MNK) */
-    LRN_TRACE_BATCHED = 10         /* batched solve kernel (work 0: not known on the host) */
+    LRN_TRACE_BATCHED = 10,        /* batched solve kernel (work 0: not known on the host) */
+    LRN_TRACE_BLOCK_UPDATE = 11,   /* update inside a block column (flops) */
+    LRN_TRACE_GEMM_FP8 = 12,       /* one-plane block-scaled FP8 trailing GEMM (flops 2MNK) */
+    LRN_TRACE_GEMM_FP8X3 = 13,     /* two-plane FP8 trailing GEMM (FP32-equivalent flops 2MNK) */
+    LRN_TRACE_GEMM_FP8X6 = 14      /* three-plane FP8 trailing GEMM (FP32-equivalent flops 2MNK) */
 };
 LRN_API int lrn_trace_start(unsigned kinds, int64_t capacity);
 LRN_API int lrn_trace_stop(void);

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 # translation units of the library (each launches only the kernels it defines)
-HIP_UNITS = ["lrn_capi.hip", "lrn_batched.hip"]
+HIP_UNITS = ["lrn_capi.hip", "lrn_batched.hip", "lrn_trsm.hip", "lrn_fp8.hip"]
 OBJ_DIR = os.path.join(HERE, "_obj")
 
 

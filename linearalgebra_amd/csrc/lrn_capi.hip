@@ -40,6 +40,8 @@
 #include "lrn_inverse.hpp"
 #include "lrn_cholesky.hpp"
 #include "lrn_bmm.hpp"
+#include "lrn_trsm_launch.hpp"
+#include "lrn_fp8_launch.hpp"
 
 namespace lrn {
 namespace {
@@ -126,6 +128,7 @@ template <typename T> struct GetrsLayout {
     Chunk* xch;  // single right-hand side: tagged exchange of solved values, one per row
     int* maps;   // net row moves of every 64-pivot block (permutation from the pivots)
     int* gate;   // nonzero: skip the solve (singular factors, or an expired factorisation wait)
+    T* dinv;     // two or more right-hand sides: inverses of the 64 x 64 diagonal blocks of L, then of U
 };
 
 long long solve_blocks(long long n) { return (n + kSolveTB - 1) / kSolveTB; }
@@ -142,6 +145,7 @@ template <typename T> GetrsLayout<T> carve_getrs(Arena& ar, long long n, long lo
     L.xch = ar.next<Chunk>(static_cast<size_t>(n));
     L.maps = ar.next<int>(static_cast<size_t>((n + 63) / 64) * kPermMapInts);
     L.gate = ar.next<int>(64);
+    L.dinv = nrhs >= 2 ? ar.next<T>(static_cast<size_t>(2 * trsm_dinv_elems(n))) : nullptr;
     return L;
 }
 
@@ -504,7 +508,9 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
 
 // Trailing update of the low-precision factorisation: 0 = bf16 operands, 1 = two bf16 planes
 // (three products), 2 = three bf16 planes (six products, FP32-level), 3 = FP32 MFMA (the
-// default: FP32-level factors, and on gfx950 faster than the six-product form of this build).
+// default: FP32-level factors, and on gfx950 faster than the six-product form of this build),
+// 4 / 5 / 6 = one / two / three block-scaled FP8 e4m3 planes (1 / 3 / 6 products).
+constexpr int kMixedUpdateModes = 7;
 std::atomic<int> g_mixed_update{3};
 
 // Panel of the low-precision factorisation: 1 = tournament pivoting, 0 = partial pivoting
@@ -548,6 +554,12 @@ int launch_trailing(const T* A, long long lda, const T* B, long long ldb, T* C, 
 {
     if constexpr (std::is_same<T, float>::value) {
         if (upd >= 0 && upd <= 2) return launch_gemm_sub_bf16(A, lda, B, ldb, C, ldc, M, N, K, s, upd + 1, cus);
+        if (upd >= 4 && upd <= 6) {
+            if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
+            const int kind = upd == 4 ? kTrGemmFp8 : (upd == 5 ? kTrGemmFp8x3 : kTrGemmFp8x6);
+            TraceScope ts(kind, 2.0 * M * N * K, s);
+            return gemm_sub_fp8_launch(A, lda, B, ldb, C, ldc, M, N, K, upd - 3, s, nullptr);
+        }
     }
     (void)upd;
     return launch_gemm_sub<T>(A, lda, B, ldb, C, ldc, M, N, K, s, cus);
@@ -901,6 +913,14 @@ std::atomic<int> g_chain_solve{0};
 // Refinement of the mixed solver: 0 = LU-IR while each step gains at least 2x, then GMRES-IR;
 // 1 = GMRES-IR right after the first LU-IR step.
 std::atomic<int> g_refinement{0};
+// Solves with at least this many right-hand sides (and at least two) use the blocked form with
+// inverted diagonal blocks (lrn_trsm.hpp); fewer use the chained substitution kernels. 0: never.
+std::atomic<int> g_multi_rhs_min{2};
+bool use_blocked_trsm(long long nrhs)
+{
+    const int m = g_multi_rhs_min.load(std::memory_order_relaxed);
+    return m > 0 && nrhs >= 2 && nrhs >= m;
+}
 
 // Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
 // without touching B (LU mode after a singular factorisation).
@@ -931,6 +951,26 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
 {
     const long long nblk = solve_blocks(n);
     if (!sticky_err) LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.err), 0u, 64, s));
+    if constexpr (std::is_same<T, TF>::value) {
+        if (L.dinv != nullptr && use_blocked_trsm(nrhs)) {
+            // blocked: P B gathered into y 64 columns at a time, L then U applied in place
+            // strip by strip with the inverted diagonal blocks, y copied back into B
+            const double tri_bytes = (0.5 * static_cast<double>(n) * n + 2.0 * n * nrhs) * sizeof(T);
+            TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
+            T* dl = L.dinv;
+            T* du = L.dinv + trsm_dinv_elems(n);
+            LRN_TRY(trsm_inv_launch<T>(true, true, false, LU, n, n, dl, skip_dev, s));
+            LRN_TRY(trsm_inv_launch<T>(false, false, false, LU, n, n, du, skip_dev, s));
+            for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
+                const long long nc = std::min<long long>(kSolveMaxCols, nrhs - c0);
+                LRN_TRY(gather_rows_launch<T>(B + c0, ldb, L.perm, L.y, nc, n, nc, skip_dev, s));
+                LRN_TRY(trsm_steps_launch<T>(true, false, LU, n, n, dl, L.y, nc, nc, skip_dev, s));
+                LRN_TRY(trsm_steps_launch<T>(false, false, LU, n, n, du, L.y, nc, nc, skip_dev, s));
+                LRN_TRY(gather_rows_launch<T>(L.y, nc, nullptr, B + c0, ldb, n, nc, skip_dev, s));
+            }
+            return LRN_OK;
+        }
+    }
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         if (nc == 1) {
@@ -1007,11 +1047,13 @@ constexpr long long kMaxOrder = 1ll << 17;
 // algorithm src/matrix_operation/cpu/native/triangular_solver_native.cpp:64-153): rows are
 // solved in 64-row blocks chained between workgroups (trsv_blocked_kernel with the
 // direction, unit-diagonal and transpose flags), 64 right-hand sides per launch.
-size_t trsm_ws_bytes(long long n)
+// Two or more right-hand sides: also the inverses of the diagonal blocks (blocked form).
+template <typename T> size_t trsm_ws_bytes(long long n, long long nrhs)
 {
     Arena ar;
     ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
     ar.next<int>(64);
+    if (nrhs >= 2) ar.next<T>(static_cast<size_t>(trsm_dinv_elems(n)));
     return ar.off;
 }
 
@@ -1039,14 +1081,21 @@ int triangular_solver_impl(int upper, int transpose, int unit, long long n, long
     if (x == nullptr || x == a || x == b) return -10;
     if (ldx < nrhs) return -11;
     if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
-    if (ws == nullptr || ws_size < trsm_ws_bytes(n)) return -12;
+    if (ws == nullptr || ws_size < trsm_ws_bytes<T>(n, nrhs)) return -12;
     Arena ar(ws, ws_size);
     unsigned* flags = ar.next<unsigned>(static_cast<size_t>(solve_blocks(n)));
     int* err = ar.next<int>(64);
+    T* dinv = nrhs >= 2 ? ar.next<T>(static_cast<size_t>(trsm_dinv_elems(n))) : nullptr;
     // rows in increasing order when op(A) is lower triangular
     const bool fwd = (upper != 0) == (transpose != 0);
     const bool u = unit != 0, t = transpose != 0;
     TraceScope ts(kTrTrsv, (0.5 * static_cast<double>(n) * n + 2.0 * static_cast<double>(n) * nrhs) * sizeof(T), s);
+    if (dinv != nullptr && use_blocked_trsm(nrhs)) {
+        // blocked: X = B, then X = op(A)^-1 X strip by strip with the inverted diagonal blocks
+        LRN_TRY(trsm_inv_launch<T>(fwd, u, t, a, lda, n, dinv, nullptr, s));
+        LRN_TRY(gather_rows_launch<T>(b, ldb, nullptr, x, ldx, n, nrhs, nullptr, s));
+        return trsm_steps_launch<T>(fwd, t, a, lda, n, dinv, x, ldx, nrhs, nullptr, s);
+    }
     LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
@@ -1180,6 +1229,17 @@ template <typename T>
 int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& G,
     const int* skip, hipStream_t s)
 {
+    if (G.dinv != nullptr && use_blocked_trsm(nrhs)) {
+        // blocked, in place on B: L (lower, non-unit) then L^T (the same triangle read transposed)
+        const double tri_bytes = (0.5 * static_cast<double>(n) * n + 2.0 * n * nrhs) * sizeof(T);
+        TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
+        T* dl = G.dinv;
+        T* dlt = G.dinv + trsm_dinv_elems(n);
+        LRN_TRY(trsm_inv_launch<T>(true, false, false, m, n, n, dl, skip, s));
+        LRN_TRY(trsm_inv_launch<T>(false, false, true, m, n, n, dlt, skip, s));
+        LRN_TRY(trsm_steps_launch<T>(true, false, m, n, n, dl, B, ldb, nrhs, skip, s));
+        return trsm_steps_launch<T>(false, true, m, n, n, dlt, B, ldb, nrhs, skip, s);
+    }
     LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
@@ -1779,7 +1839,7 @@ bool mixed_verbose()
 int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, double tol, int max_iter, int m,
     void* ws, size_t ws_size, lrn_mixed_stats* stats, hipStream_t s, int upd = -1)
 {
-    if (upd < 0 || upd > 3) upd = g_mixed_update.load();
+    if (upd < 0 || upd >= kMixedUpdateModes) upd = g_mixed_update.load();
     const bool fallback = ws_size >= mixed_ws_bytes(n, m, true);
     if (ws_size < mixed_ws_bytes(n, m, false)) return -9;
     Arena ar(ws, ws_size);
@@ -2121,12 +2181,16 @@ LRN_API int lrn_triangular_solver_workspace_size_f64(int64_t n, int64_t nrhs, si
     if (n <= 0) return -1;
     if (nrhs <= 0) return -2;
     if (bytes == nullptr) return -3;
-    *bytes = lrn::trsm_ws_bytes(n);
+    *bytes = lrn::trsm_ws_bytes<double>(n, nrhs);
     return LRN_OK;
 }
 LRN_API int lrn_triangular_solver_workspace_size_f32(int64_t n, int64_t nrhs, size_t* bytes)
 {
-    return lrn_triangular_solver_workspace_size_f64(n, nrhs, bytes);
+    if (n <= 0) return -1;
+    if (nrhs <= 0) return -2;
+    if (bytes == nullptr) return -3;
+    *bytes = lrn::trsm_ws_bytes<float>(n, nrhs);
+    return LRN_OK;
 }
 LRN_API int lrn_triangular_solver_f64(int upper, int transpose, int unit, int64_t n, int64_t nrhs, const double* a,
     int64_t lda, const double* b, int64_t ldb, double* x, int64_t ldx, void* workspace, size_t workspace_size, void* stream)
@@ -2307,6 +2371,9 @@ LRN_API int lrn_debug_gemm_sub(int dtype, const void* a, int64_t lda, const void
     if (dtype >= 2 && dtype <= 4)
         return launch_gemm_sub_bf16(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
             static_cast<float*>(c), ldc, m, n, k, s, dtype - 1);
+    if (dtype >= 5 && dtype <= 7)
+        return gemm_sub_fp8_launch(static_cast<const float*>(a), lda, static_cast<const float*>(b), ldb,
+            static_cast<float*>(c), ldc, m, n, k, dtype - 4, s, nullptr);
     return -1;
 }
 
@@ -2371,7 +2438,7 @@ LRN_API int lrn_gesv_mixed_batched_f64(const double* a, const int64_t* a_offset,
     if (workspace == nullptr) return -13;
     if (stats == nullptr) return -15;
     for (int64_t i = 0; i < count; ++i)
-        if (n[i] <= 0 || n[i] > kMaxOrder || (update_mode != nullptr && (update_mode[i] < 0 || update_mode[i] > 3)))
+        if (n[i] <= 0 || n[i] > kMaxOrder || (update_mode != nullptr && (update_mode[i] < 0 || update_mode[i] >= kMixedUpdateModes)))
             return -3;
     for (int64_t i = 0; i < count; ++i) {
         const int st = gesv_mixed_impl(a + a_offset[i], n[i], b + b_offset[i], x + x_offset[i], tol, max_iter,
@@ -2452,7 +2519,7 @@ LRN_API int lrn_trace_read(int kind, int64_t* launches, double* milliseconds, do
 
 LRN_API int lrn_set_mixed_update(int mode)
 {
-    if (mode < 0 || mode > 3) return -1;
+    if (mode < 0 || mode >= kMixedUpdateModes) return -1;
     g_mixed_update.store(mode);
     return LRN_OK;
 }
@@ -2488,6 +2555,13 @@ LRN_API int lrn_set_refinement(int mode)
 {
     if (mode < 0 || mode > 1) return -1;
     g_refinement.store(mode);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_multi_rhs_solver(int min_rhs)
+{
+    if (min_rhs < 0) return -1;
+    g_multi_rhs_min.store(min_rhs);
     return LRN_OK;
 }
 

@@ -205,4 +205,33 @@ __device__ __forceinline__ void wave_argmax_ordered(double& key, long long& row)
     row = r;
 }
 
+template <typename T> __device__ __forceinline__ T read_lane_t(T v, int lane);
+template <> __device__ __forceinline__ double read_lane_t<double>(double v, int lane) { return read_lane_f64(v, lane); }
+template <> __device__ __forceinline__ float read_lane_t<float>(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+
+template <typename T> struct Mfma16;
+// Operands of both: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k. Result element v of
+// lane l holds D[row(l, v)][l % 16]; the row order differs between the two instructions
+// (measured on gfx950 with one-hot operands).
+template <> struct Mfma16<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct Mfma16<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 }  // namespace lrn

@@ -675,27 +675,6 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
 
 // ---------------------------------------------------------------------------------- GEMM
 
-template <typename T> struct Mfma16;
-// Operands of both: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k. Result element v of
-// lane l holds D[row(l, v)][l % 16]; the row order differs between the two instructions
-// (measured on gfx950 with one-hot operands).
-template <> struct Mfma16<double> {
-    typedef double acc_t __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
-};
-template <> struct Mfma16<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
 // C[M x N] -= A[M x K] * B[K x N], all row-major (leading dimensions lda, ldb, ldc).
 // 256 threads = 4 waves in a 2 x 2 arrangement; each wave owns a (BM/2) x (BN/2) block of
 // 16 x 16 MFMA tiles. A and B tiles are staged k-major in LDS so each lane's MFMA operand
@@ -805,7 +784,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, 
 
 // Zero-fill of small control areas (a kernel rather than hipMemsetAsync, so the sequence
 // stays valid inside a captured graph).
-__global__ void fill_u32_kernel(unsigned* p, unsigned v, long long count)
+static __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long count)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;

@@ -23,7 +23,7 @@ constexpr int kPermLdsMax = 36864;    // largest n whose permutation is built in
 // in order on a label array (LDS for n <= kPermLdsMax), prefetching the next block's moves.
 constexpr int kPermMapInts = 1 + 4 * 64;
 
-__global__ __launch_bounds__(64) void pivot_block_maps_kernel(const long long* __restrict__ piv, long long n,
+static __global__ __launch_bounds__(64) void pivot_block_maps_kernel(const long long* __restrict__ piv, long long n,
     int* __restrict__ maps)
 {
     __shared__ long long pivs[64];
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(64) void pivot_block_maps_kernel(const long long* _
     panel_rowmap(pivs, k, w, 128, maps + static_cast<size_t>(blockIdx.x) * kPermMapInts);
 }
 
-__global__ __launch_bounds__(256) void perm_compose_kernel(const int* __restrict__ maps, long long n, long long nblocks,
+static __global__ __launch_bounds__(256) void perm_compose_kernel(const int* __restrict__ maps, long long n, long long nblocks,
     int* __restrict__ perm_out, int* __restrict__ perm_tmp)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -73,13 +73,6 @@ __global__ __launch_bounds__(256) void perm_compose_kernel(const int* __restrict
         src = nsrc;
     }
     for (long long i = t; i < n; i += 256) perm_out[i] = perm[i];
-}
-
-template <typename T> __device__ __forceinline__ T read_lane_t(T v, int lane);
-template <> __device__ __forceinline__ double read_lane_t<double>(double v, int lane) { return read_lane_f64(v, lane); }
-template <> __device__ __forceinline__ float read_lane_t<float>(float v, int lane)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // One workgroup per 64-row block; block b waits (bounded) for the blocks it depends on,

@@ -27,7 +27,10 @@ enum TraceKind {
     kTrGemmBf16x6 = 9,   // gemm_sub_bf16_kernel<3>  (three planes), FP32-equivalent flops 2MNK
     kTrBatched = 10,     // batched_gesv_kernel (work not known on the host: 0)
     kTrBcUpdate = 11,    // update inside a block column (bc_update_kernel / 64x64 gemm_sub_kernel), flops
-    kTrKinds = 12
+    kTrGemmFp8 = 12,     // gemm_sub_fp8_kernel<1> (one block-scaled e4m3 plane), flops 2MNK
+    kTrGemmFp8x3 = 13,   // gemm_sub_fp8_kernel<2> (two planes, three products), FP32-equivalent flops 2MNK
+    kTrGemmFp8x6 = 14,   // gemm_sub_fp8_kernel<3> (three planes, six products), FP32-equivalent flops 2MNK
+    kTrKinds = 15
 };
 
 struct TraceRec {

@@ -1,0 +1,31 @@
+// Launch interface of the blocked many-right-hand-side triangular solve, whose kernels live in
+// their own translation unit (lrn_trsm.hip). Internal to the library (hidden visibility).
+// Every function returns 0 or a positive status (HIP failure); every kernel returns at once
+// when `skip` is non-null and *skip is nonzero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lrn {
+
+// Elements of T holding the inverses of the 64 x 64 diagonal blocks of an n x n operator.
+long long trsm_dinv_elems(long long n);
+
+// Inverts every 64 x 64 diagonal block of op(A) (lower: op(A) lower triangular; unit: implicit
+// unit diagonal; trans: op(A) = A^T) into dinv (trsm_dinv_elems(n) elements).
+template <typename T>
+int trsm_inv_launch(bool lower, bool unit, bool trans, const T* a, long long lda, long long n, T* dinv,
+    const int* skip, hipStream_t s);
+
+// X = op(A)^-1 X in place (X n x nrhs, row stride ldx) with the inverses from trsm_inv_launch
+// (same lower / trans flags): one launch per 64-row strip.
+template <typename T>
+int trsm_steps_launch(bool lower, bool trans, const T* a, long long lda, long long n, const T* dinv, T* x,
+    long long ldx, long long nrhs, const int* skip, hipStream_t s);
+
+// Y[i][c] = B[perm[i]][c] (B[i][c] when perm is null) for i < n, c < nc.
+template <typename T>
+int gather_rows_launch(const T* b, long long ldb, const int* perm, T* y, long long ldy, long long n, long long nc,
+    const int* skip, hipStream_t s);
+
+}  // namespace lrn

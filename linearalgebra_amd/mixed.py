@@ -46,7 +46,8 @@ def workspace_size(n: int, restart: int = 30, fallback: bool = True) -> int:
 def set_mixed_update(mode: int) -> None:
     """Trailing update of the low-precision factorisation (process-wide): 0, 1, 2 = FP32
     operands split into 1, 2, 3 bf16 planes on the bf16 matrix cores (2 gives FP32-level
-    factors); 3 = FP32 matrix instructions (the default)."""
+    factors); 3 = FP32 matrix instructions (the default); 4, 5, 6 = 1, 2, 3 planes of FP8
+    e4m3 with one power-of-two scale per 32 K elements (block-scaled FP8 matrix cores)."""
     _check(_lib.load().lrn_set_mixed_update(int(mode)), "set_mixed_update")
 
 

@@ -63,3 +63,53 @@ def test_triangular_invalid_arguments(dev):
         la.triangular_solver.apply(False, False, False, 4, 2, a, 4, b, 2, b, 2)
     with pytest.raises(ValueError):
         la.triangular_solver.workspace_size(2, -1)
+
+
+def _multi_rhs_solver(min_rhs):
+    from linearalgebra_amd import _lib
+    assert _lib.load().lrn_set_multi_rhs_solver(int(min_rhs)) == 0
+
+
+@pytest.mark.parametrize("n,nrhs", [(130, 2), (1000, 64), (4100, 130)])
+@pytest.mark.parametrize("upper,transpose,unit", [(False, False, True), (True, False, False), (False, True, False),
+                                                  (True, True, True)])
+def test_triangular_blocked_matches_chained_and_oracle(n, nrhs, upper, transpose, unit, dev):
+    """The blocked form (inverted 64 x 64 diagonal blocks, one matrix-core update per strip;
+    nrhs > 64 covers several column groups, n = 4100 a partial last strip) against the chained
+    substitution and the oracle, on a well-conditioned triangle: both within 1e-12."""
+    rng = np.random.default_rng(3 * n + nrhs)
+    a = 0.2 * rng.standard_normal((n, n)) / np.sqrt(n) + np.diag(2.0 + rng.random(n))
+    # NaN outside the triangle op(A) uses: neither form may read it
+    lower_stored = (not upper)
+    mask = np.tril(np.ones((n, n), dtype=bool)) if lower_stored else np.triu(np.ones((n, n), dtype=bool))
+    a_nan = np.where(mask, a, np.nan)
+    b = rng.standard_normal((n, nrhs))
+    try:
+        _, _, xb = _solve(upper, transpose, unit, n, nrhs, a_nan, n, b, nrhs, nrhs + 3, dev)
+        _multi_rhs_solver(0)
+        _, _, xc = _solve(upper, transpose, unit, n, nrhs, a_nan, n, b, nrhs, nrhs + 3, dev)
+    finally:
+        _multi_rhs_solver(2)
+    assert (xb[:, nrhs:] == -123).all() and (xc[:, nrhs:] == -123).all()
+    xb, xc = xb[:, :nrhs], xc[:, :nrhs]
+    assert np.isfinite(xb).all()
+    d_bc = np.abs(xb - xc).max() / np.abs(xc).max()
+    msg = f"trsm blocked n={n} nrhs={nrhs} upper={upper} transpose={transpose} unit={unit}: vs chained {d_bc:.2e}"
+    if n <= 1000:
+        ref = O.trsm(upper, transpose, unit, a, b)
+        d_o = np.abs(xb - ref).max() / np.abs(ref).max()
+        msg += f", vs oracle {d_o:.2e}"
+        assert d_o <= 1e-12
+    print(msg, flush=True)
+    assert d_bc <= 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 4e-6)])
+def test_triangular_known_answers_chained(dtype, tol, dev):
+    """The reference suite's known answers through the chained kernels as well (the default
+    sends every case with two or more right-hand sides through the blocked form)."""
+    try:
+        _multi_rhs_solver(0)
+        test_triangular_known_answers(dtype, tol, dev)
+    finally:
+        _multi_rhs_solver(2)
