@@ -338,6 +338,10 @@ LRN_API int lrn_set_refinement(int mode);
  * inverted, then one matrix-core update launch per 64-row strip); fewer, or min_rhs = 0, the
  * chained substitution kernels. Default 2. Process-wide. */
 LRN_API int lrn_set_multi_rhs_solver(int min_rhs);
+/* Kernel of that blocked form (process-wide): 1 = one launch per 64 right-hand sides, the
+ * 64-row strips chained through flags between workgroups (bounded waits; the default, used when
+ * all of its workgroups can be resident at once), 0 = one launch per 64-row strip. */
+LRN_API int lrn_set_multi_rhs_kernel(int kind);
 
 /* ---- kernel test hooks (no reference counterpart) -------------------------------------
  * Single-kernel entry points used by the kernel-level unit tests; dtype 0 = f64, 1 = f32,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "lrn_set_block_update": [_int],
     "lrn_set_refinement": [_int],
     "lrn_set_multi_rhs_solver": [_int],
+    "lrn_set_multi_rhs_kernel": [_int],
     "lrn_set_mixed_update": [_int],
     "lrn_gesv_mixed_batched_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, ctypes.c_double, _int, _int, _vp, _sz,
                                    _vp, _vp],

@@ -380,10 +380,22 @@ __global__ __launch_bounds__(kBatThreads, 2) void batched_gesv_kernel(T* __restr
             const int rtiles = (mr + 31) / 32;
             for (int cb = 0; cb < nc; cb += kBatUC) {
                 __syncthreads();  // U12 written (first chunk) / the previous chunk's tiles done
-                for (int idx = tid; idx < NB * kBatUC; idx += kBatThreads) {
-                    const int pp = idx / kBatUC, cc = idx - pp * kBatUC;
-                    S.us[pp][cc] = (pp < w && cb + cc < nc) ? A[static_cast<long long>(S.phys[k + pp]) * n + kw + cb + cc]
-                                                            : T(0);
+                {
+                    // the thread's loads all in flight before any is written to LDS
+                    constexpr int kPer = NB * kBatUC / kBatThreads;
+                    T uv[kPer];
+#pragma unroll
+                    for (int e = 0; e < kPer; ++e) {
+                        const int idx = tid + kBatThreads * e;
+                        const int pp = idx / kBatUC, cc = idx - pp * kBatUC;
+                        uv[e] = (pp < w && cb + cc < nc) ? A[static_cast<long long>(S.phys[k + pp]) * n + kw + cb + cc]
+                                                         : T(0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < kPer; ++e) {
+                        const int idx = tid + kBatThreads * e;
+                        S.us[idx / kBatUC][idx % kBatUC] = uv[e];
+                    }
                 }
                 __syncthreads();
                 const int ctiles = min(2, (nc - cb + 31) / 32);

@@ -921,6 +921,29 @@ bool use_blocked_trsm(long long nrhs)
     const int m = g_multi_rhs_min.load(std::memory_order_relaxed);
     return m > 0 && nrhs >= 2 && nrhs >= m;
 }
+// Kernel of the blocked form: 0 = one launch per 64-row strip (trsm_step_kernel), 1 = one
+// launch per 64 columns with the strips chained through flags (trsm_chain_kernel; falls back
+// to 0 when its workgroups cannot all be resident).
+std::atomic<int> g_multi_rhs_kernel{1};
+
+// X = op(A)^-1 X (X n x nrhs, stride ldx) with the diagonal-block inverses in dinv. flags:
+// solve_blocks(n) words, err: the error word (an expired chained wait raises it).
+template <typename T>
+int trsm_apply(bool lower, bool trans, const T* a, long long lda, long long n, const T* dinv, T* x, long long ldx,
+    long long nrhs, unsigned* flags, int* err, const int* skip, hipStream_t s)
+{
+    if (g_multi_rhs_kernel.load(std::memory_order_relaxed) == 1) {
+        int rc = LRN_OK;
+        for (long long c0 = 0; c0 < nrhs && rc == LRN_OK; c0 += kSolveMaxCols) {
+            const long long nc = std::min<long long>(kSolveMaxCols, nrhs - c0);
+            LRN_TRY(launch_fill(flags, 0u, solve_blocks(n), s));
+            rc = trsm_chain_launch<T>(lower, trans, a, lda, n, dinv, x + c0, ldx, nc, flags, spin_limit(), err, skip, s);
+            if (rc == LRN_E_UNSUPPORTED && c0 != 0) return LRN_E_INTERNAL;  // residency cannot change between chunks
+        }
+        if (rc != LRN_E_UNSUPPORTED) return rc;
+    }
+    return trsm_steps_launch<T>(lower, trans, a, lda, n, dinv, x, ldx, nrhs, skip, s);
+}
 
 // Solve with factors already in place. skip_dev: if non-null and nonzero, the kernels return
 // without touching B (LU mode after a singular factorisation).
@@ -964,10 +987,11 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
             for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
                 const long long nc = std::min<long long>(kSolveMaxCols, nrhs - c0);
                 LRN_TRY(gather_rows_launch<T>(B + c0, ldb, L.perm, L.y, nc, n, nc, skip_dev, s));
-                LRN_TRY(trsm_steps_launch<T>(true, false, LU, n, n, dl, L.y, nc, nc, skip_dev, s));
-                LRN_TRY(trsm_steps_launch<T>(false, false, LU, n, n, du, L.y, nc, nc, skip_dev, s));
+                LRN_TRY(trsm_apply<T>(true, false, LU, n, n, dl, L.y, nc, nc, L.flags_f, L.err, skip_dev, s));
+                LRN_TRY(trsm_apply<T>(false, false, LU, n, n, du, L.y, nc, nc, L.flags_b, L.err, skip_dev, s));
                 LRN_TRY(gather_rows_launch<T>(L.y, nc, nullptr, B + c0, ldb, n, nc, skip_dev, s));
             }
+            if (!sticky_err) LRN_TRY(launch_poison<T>(L.err, B, n, nrhs, ldb, s));
             return LRN_OK;
         }
     }
@@ -1090,13 +1114,14 @@ int triangular_solver_impl(int upper, int transpose, int unit, long long n, long
     const bool fwd = (upper != 0) == (transpose != 0);
     const bool u = unit != 0, t = transpose != 0;
     TraceScope ts(kTrTrsv, (0.5 * static_cast<double>(n) * n + 2.0 * static_cast<double>(n) * nrhs) * sizeof(T), s);
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
     if (dinv != nullptr && use_blocked_trsm(nrhs)) {
         // blocked: X = B, then X = op(A)^-1 X strip by strip with the inverted diagonal blocks
         LRN_TRY(trsm_inv_launch<T>(fwd, u, t, a, lda, n, dinv, nullptr, s));
         LRN_TRY(gather_rows_launch<T>(b, ldb, nullptr, x, ldx, n, nrhs, nullptr, s));
-        return trsm_steps_launch<T>(fwd, t, a, lda, n, dinv, x, ldx, nrhs, nullptr, s);
+        LRN_TRY(trsm_apply<T>(fwd, t, a, lda, n, dinv, x, ldx, nrhs, flags, err, nullptr, s));
+        return launch_poison<T>(err, x, n, nrhs, ldx, s);
     }
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         LRN_TRY(launch_fill(flags, 0u, solve_blocks(n), s));
@@ -1229,6 +1254,7 @@ template <typename T>
 int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, const GetrsLayout<T>& G,
     const int* skip, hipStream_t s)
 {
+    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
     if (G.dinv != nullptr && use_blocked_trsm(nrhs)) {
         // blocked, in place on B: L (lower, non-unit) then L^T (the same triangle read transposed)
         const double tri_bytes = (0.5 * static_cast<double>(n) * n + 2.0 * n * nrhs) * sizeof(T);
@@ -1237,10 +1263,10 @@ int chol_solve(const T* m, long long n, T* B, long long nrhs, long long ldb, con
         T* dlt = G.dinv + trsm_dinv_elems(n);
         LRN_TRY(trsm_inv_launch<T>(true, false, false, m, n, n, dl, skip, s));
         LRN_TRY(trsm_inv_launch<T>(false, false, true, m, n, n, dlt, skip, s));
-        LRN_TRY(trsm_steps_launch<T>(true, false, m, n, n, dl, B, ldb, nrhs, skip, s));
-        return trsm_steps_launch<T>(false, true, m, n, n, dlt, B, ldb, nrhs, skip, s);
+        LRN_TRY(trsm_apply<T>(true, false, m, n, n, dl, B, ldb, nrhs, G.flags_f, G.err, skip, s));
+        LRN_TRY(trsm_apply<T>(false, true, m, n, n, dlt, B, ldb, nrhs, G.flags_b, G.err, skip, s));
+        return launch_poison<T>(G.err, B, n, nrhs, ldb, s);
     }
-    LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(G.err), 0u, 64, s));
     for (long long c0 = 0; c0 < nrhs; c0 += kSolveMaxCols) {
         const int nc = static_cast<int>(std::min<long long>(kSolveMaxCols, nrhs - c0));
         LRN_TRY(launch_fill(G.flags_f, 0u, solve_blocks(n), s));
@@ -2562,6 +2588,13 @@ LRN_API int lrn_set_multi_rhs_solver(int min_rhs)
 {
     if (min_rhs < 0) return -1;
     g_multi_rhs_min.store(min_rhs);
+    return LRN_OK;
+}
+
+LRN_API int lrn_set_multi_rhs_kernel(int kind)
+{
+    if (kind < 0 || kind > 1) return -1;
+    g_multi_rhs_kernel.store(kind);
     return LRN_OK;
 }
 

@@ -79,10 +79,26 @@ __global__ __launch_bounds__(64) void chol_rows_kernel(T* __restrict__ A, long l
     const int t = threadIdx.x;
     const long long r0 = k + bs + static_cast<long long>(blockIdx.x) * 64;
     const int rows = static_cast<int>(n - r0 < 64 ? n - r0 : 64);
-    for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
-        const int i = idx / kCholNB, j = idx - i * kCholNB;
-        l11[i][j] = (i < bs && j <= i) ? A[(k + i) * lda + k + j] : T(0);
-        tile[i][j] = (i < rows && j < bs) ? A[(r0 + i) * lda + k + j] : T(0);
+    // loads in batches of 16 per thread, all in flight before any is written to LDS (a rolled
+    // loop would wait for each load in turn: 64 round trips per thread)
+    constexpr int kBatch = 16;
+#pragma unroll
+    for (int base = 0; base < kCholNB * kCholNB; base += 64 * kBatch) {
+        T lv[kBatch], tv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = base + t + 64 * u;
+            const int i = idx / kCholNB, j = idx - i * kCholNB;
+            lv[u] = (i < bs && j <= i) ? A[(k + i) * lda + k + j] : T(0);
+            tv[u] = (i < rows && j < bs) ? A[(r0 + i) * lda + k + j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = base + t + 64 * u;
+            const int i = idx / kCholNB, j = idx - i * kCholNB;
+            l11[i][j] = lv[u];
+            tile[i][j] = tv[u];
+        }
     }
     __syncthreads();
     if (t < rows) {
@@ -103,11 +119,13 @@ __global__ __launch_bounds__(64) void chol_rows_kernel(T* __restrict__ A, long l
         for (int p = 0; p < kCholNB; ++p) tile[t][p] = x[p];
     }
     __syncthreads();
+#pragma unroll 16
     for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
         const int i = idx / kCholNB, j = idx - i * kCholNB;
         if (i < rows && j < bs) A[(r0 + i) * lda + k + j] = tile[i][j];
     }
     // transposed copy: row j of Wt holds column j of the solved rows
+#pragma unroll 16
     for (int idx = t; idx < kCholNB * kCholNB; idx += 64) {
         const int j = idx / kCholNB, i = idx - j * kCholNB;
         if (i < rows && j < bs) Wt[j * ldw + (r0 - k - bs) + i] = tile[i][j];

@@ -115,15 +115,40 @@ __global__ __launch_bounds__(256) void gemm_sub_fp8_kernel(const float* __restri
 
     for (int k0 = 0; k0 < K; k0 += kF8BK) {
         __syncthreads();
-        for (int idx = tid; idx < kF8BM * kF8BK; idx += 256) {
-            const int r = idx / kF8BK, k = idx % kF8BK;
-            const long long gr = row0 + r;
-            As[r][k] = (gr < M && k0 + k < K) ? A[gr * lda + k0 + k] : 0.0f;
+        // operand tiles in batches of 16 loads per thread, all in flight before any is written
+        // to LDS (a rolled loop would wait for each load in turn)
+        constexpr int kBatch = 16;
+#pragma unroll
+        for (int base = 0; base < kF8BM * kF8BK; base += 256 * kBatch) {
+            float v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int idx = base + tid + 256 * u;
+                const int r = idx / kF8BK, k = idx % kF8BK;
+                const long long gr = row0 + r;
+                v[u] = (gr < M && k0 + k < K) ? A[gr * lda + k0 + k] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int idx = base + tid + 256 * u;
+                As[idx / kF8BK][idx % kF8BK] = v[u];
+            }
         }
-        for (int idx = tid; idx < kF8BK * kF8BN; idx += 256) {
-            const int k = idx / kF8BN, c = idx % kF8BN;
-            const long long gc = col0 + c;
-            Bs[c][k] = (gc < N && k0 + k < K) ? B[static_cast<long long>(k0 + k) * ldb + gc] : 0.0f;
+#pragma unroll
+        for (int base = 0; base < kF8BK * kF8BN; base += 256 * kBatch) {
+            float v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int idx = base + tid + 256 * u;
+                const int k = idx / kF8BN, c = idx % kF8BN;
+                const long long gc = col0 + c;
+                v[u] = (gc < N && k0 + k < K) ? B[static_cast<long long>(k0 + k) * ldb + gc] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int idx = base + tid + 256 * u;
+                Bs[idx % kF8BN][idx / kF8BN] = v[u];
+            }
         }
         __syncthreads();
 

@@ -295,24 +295,57 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long
     const int cnt = rowmap[0];
     const int* dst = rowmap + 1;
     const int* src = rowmap + 1 + W2;
-    for (int idx = tid; idx < cnt * kSwapCols; idx += 256) {
-        const int q = idx / kSwapCols, c = idx - q * kSwapCols;
-        if (c < ncols) buf[q][c] = A[static_cast<long long>(src[q]) * lda + cb + c];
+    // every load of a phase is issued before the first of them is used: fully unrolled loops
+    // over the thread's share, through registers (a rolled loop waits for each load in turn)
+    constexpr int kMoves = 2 * W * kSwapCols / 256;
+    {
+        T v[kMoves];
+#pragma unroll
+        for (int t = 0; t < kMoves; ++t) {
+            const int idx = tid + 256 * t;
+            const int q = idx / kSwapCols, c = idx - q * kSwapCols;
+            v[t] = (q < cnt && c < ncols) ? A[static_cast<long long>(src[q]) * lda + cb + c] : T(0);
+        }
+#pragma unroll
+        for (int t = 0; t < kMoves; ++t) {
+            const int idx = tid + 256 * t;
+            buf[idx / kSwapCols][idx % kSwapCols] = v[t];
+        }
     }
     __syncthreads();
-    for (int idx = tid; idx < cnt * kSwapCols; idx += 256) {
+#pragma unroll
+    for (int t = 0; t < kMoves; ++t) {
+        const int idx = tid + 256 * t;
         const int q = idx / kSwapCols, c = idx - q * kSwapCols;
-        if (c < ncols) A[static_cast<long long>(dst[q]) * lda + cb + c] = buf[q][c];
+        if (q < cnt && c < ncols) A[static_cast<long long>(dst[q]) * lda + cb + c] = buf[q][c];
     }
     if (!right) return;  // uniform over the workgroup
     __syncthreads();
-    for (int idx = tid; idx < W * W; idx += 256) {
-        const int r = idx / W, c = idx % W;
-        L11[r][c] = (r < w && c < w) ? A[(k + r) * lda + k + c] : T(0);
-    }
-    for (int idx = tid; idx < W * kSwapCols; idx += 256) {
-        const int r = idx / kSwapCols, c = idx % kSwapCols;
-        X[r][c] = (r < w && c < ncols) ? A[(k + r) * lda + cb + c] : T(0);
+    {
+        constexpr int kL = (W * W + 255) / 256, kX = (W * kSwapCols + 255) / 256;
+        T lv[kL], xv[kX];
+#pragma unroll
+        for (int t = 0; t < kL; ++t) {
+            const int idx = tid + 256 * t;
+            const int r = idx / W, c = idx % W;
+            lv[t] = (idx < W * W && r < w && c < w) ? A[(k + r) * lda + k + c] : T(0);
+        }
+#pragma unroll
+        for (int t = 0; t < kX; ++t) {
+            const int idx = tid + 256 * t;
+            const int r = idx / kSwapCols, c = idx % kSwapCols;
+            xv[t] = (idx < W * kSwapCols && r < w && c < ncols) ? A[(k + r) * lda + cb + c] : T(0);
+        }
+#pragma unroll
+        for (int t = 0; t < kL; ++t) {
+            const int idx = tid + 256 * t;
+            if (idx < W * W) L11[idx / W][idx % W] = lv[t];
+        }
+#pragma unroll
+        for (int t = 0; t < kX; ++t) {
+            const int idx = tid + 256 * t;
+            if (idx < W * kSwapCols) X[idx / kSwapCols][idx % kSwapCols] = xv[t];
+        }
     }
     __syncthreads();
     if (tid < ncols) {
@@ -550,12 +583,26 @@ __global__ __launch_bounds__(256) void laswp_block_kernel(T* __restrict__ A, lon
         }
     }
     const int ns = nslots;
-    for (int idx = tid; idx < ns * SC; idx += 256) {
-        const int sl = idx / SC, c = idx - sl * SC;
-        const long long t = s0 + c;
-        if (t < total) {
-            const long long col = (t < la) ? c0a + t : c0b + (t - la);
-            buf[sl][c] = A[static_cast<long long>(rowof[sl]) * lda + col];
+    {
+        // all of the thread's gathers in flight at once (unrolled, through registers)
+        constexpr int kG = kSwapBlkRows * SC / 256;
+        T v[kG];
+#pragma unroll
+        for (int g = 0; g < kG; ++g) {
+            const int idx = tid + 256 * g;
+            const int sl = idx / SC, c = idx - sl * SC;
+            const long long t = s0 + c;
+            v[g] = T(0);
+            if (sl < ns && t < total) {
+                const long long col = (t < la) ? c0a + t : c0b + (t - la);
+                v[g] = A[static_cast<long long>(rowof[sl]) * lda + col];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kG; ++g) {
+            const int idx = tid + 256 * g;
+            const int sl = idx / SC, c = idx - sl * SC;
+            if (sl < ns) buf[sl][c] = v[g];
         }
     }
     __syncthreads();

@@ -73,9 +73,22 @@ __global__ __launch_bounds__(64) void strip_trsm_kernel(const T* __restrict__ A,
     T* __restrict__ X, long long ldx, long long ncols)
 {
     __shared__ T tri[kStripRows][kStripRows + 1];
-    for (int idx = threadIdx.x; idx < kStripRows * kStripRows; idx += 64) {
-        const int i = idx / kStripRows, j = idx - i * kStripRows;
-        tri[i][j] = (i < w && j < w) ? A[(r0 + i) * lda + r0 + j] : T(0);
+    // loads in batches of 16 per thread, all in flight before any is written to LDS
+    constexpr int kBatch = 16;
+#pragma unroll
+    for (int base = 0; base < kStripRows * kStripRows; base += 64 * kBatch) {
+        T v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = base + static_cast<int>(threadIdx.x) + 64 * u;
+            const int i = idx / kStripRows, j = idx - i * kStripRows;
+            v[u] = (i < w && j < w) ? A[(r0 + i) * lda + r0 + j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = base + static_cast<int>(threadIdx.x) + 64 * u;
+            tri[idx / kStripRows][idx % kStripRows] = v[u];
+        }
     }
     __syncthreads();
     const long long col = static_cast<long long>(blockIdx.x) * 64 + threadIdx.x;

@@ -68,6 +68,47 @@ int trsm_steps_launch(bool lower, bool trans, const T* a, long long lda, long lo
                  : steps_go<T, false, false>(a, lda, n, dinv, x, ldx, nrhs, skip, s);
 }
 
+long long trsm_flag_count(long long n, long long nrhs)
+{
+    return (n + kTsRows - 1) / kTsRows * ((nrhs + kTsRows - 1) / kTsRows);
+}
+
+namespace {
+
+template <typename T, bool kLower, bool kTrans>
+int chain_go(const T* a, long long lda, long long n, const T* dinv, T* x, long long ldx, long long nrhs,
+    unsigned* flags, unsigned spin_limit, int* err, const int* skip, hipStream_t s)
+{
+    auto kern = trsm_chain_kernel<T, kLower, kTrans>;
+    const long long nblk = (n + kTsRows - 1) / kTsRows;
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LRN_E_HIP;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 256, 0) != hipSuccess)
+        return LRN_E_HIP;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return LRN_E_HIP;
+    if (nblk > static_cast<long long>(per_cu) * cus) return LRN_E_UNSUPPORTED;
+    for (long long c0 = 0, q = 0; c0 < nrhs; c0 += kTsRows, ++q) {
+        const long long nc = std::min<long long>(kTsRows, nrhs - c0);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, kLower, kTrans>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, a,
+            lda, n, dinv, x + c0, ldx, nc, flags + q * nblk, spin_limit, err, skip);
+        if (hipGetLastError() != hipSuccess) return LRN_E_HIP;
+    }
+    return LRN_OK;
+}
+
+}  // namespace
+
+template <typename T>
+int trsm_chain_launch(bool lower, bool trans, const T* a, long long lda, long long n, const T* dinv, T* x,
+    long long ldx, long long nrhs, unsigned* flags, unsigned spin_limit, int* err, const int* skip, hipStream_t s)
+{
+    if (lower)
+        return trans ? chain_go<T, true, true>(a, lda, n, dinv, x, ldx, nrhs, flags, spin_limit, err, skip, s)
+                     : chain_go<T, true, false>(a, lda, n, dinv, x, ldx, nrhs, flags, spin_limit, err, skip, s);
+    return trans ? chain_go<T, false, true>(a, lda, n, dinv, x, ldx, nrhs, flags, spin_limit, err, skip, s)
+                 : chain_go<T, false, false>(a, lda, n, dinv, x, ldx, nrhs, flags, spin_limit, err, skip, s);
+}
+
 template <typename T>
 int gather_rows_launch(const T* b, long long ldb, const int* perm, T* y, long long ldy, long long n, long long nc,
     const int* skip, hipStream_t s)
@@ -86,6 +127,10 @@ template int trsm_steps_launch<double>(bool, bool, const double*, long long, lon
     long long, long long, const int*, hipStream_t);
 template int trsm_steps_launch<float>(bool, bool, const float*, long long, long long, const float*, float*, long long,
     long long, const int*, hipStream_t);
+template int trsm_chain_launch<double>(bool, bool, const double*, long long, long long, const double*, double*,
+    long long, long long, unsigned*, unsigned, int*, const int*, hipStream_t);
+template int trsm_chain_launch<float>(bool, bool, const float*, long long, long long, const float*, float*, long long,
+    long long, unsigned*, unsigned, int*, const int*, hipStream_t);
 template int gather_rows_launch<double>(const double*, long long, const int*, double*, long long, long long, long long,
     const int*, hipStream_t);
 template int gather_rows_launch<float>(const float*, long long, const int*, float*, long long, long long, long long,

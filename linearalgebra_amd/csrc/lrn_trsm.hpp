@@ -33,6 +33,56 @@ __device__ __forceinline__ T op_coef(const T* __restrict__ A, long long lda, lon
     return kTrans ? A[j * lda + i] : A[i * lda + j];
 }
 
+// Tile loads: the 16 elements of each thread are loaded into registers by a fully unrolled
+// loop before any of them is written to LDS, so the 16 loads are in flight together (a rolled
+// loop issues one load, waits for it and writes LDS, sixteen times in a row).
+constexpr int kTsPer = kTsRows * kTsRows / 256;
+
+// dst[k][i] = op(A)[r0 + i][k0 + k] (zero outside rows x krows), read along A's contiguous index
+template <typename T, bool kTrans>
+__device__ __forceinline__ void ts_load_op_tile(T (*dst)[kTsRows + 2], const T* __restrict__ A, long long lda,
+    long long r0, int rows, long long k0, int krows, int tid)
+{
+    T v[kTsPer];
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) {
+        const int idx = tid + 256 * q;
+        const int i = kTrans ? (idx & 63) : (idx >> 6);
+        const int k = kTrans ? (idx >> 6) : (idx & 63);
+        v[q] = (i < rows && k < krows) ? op_coef<T, kTrans>(A, lda, r0 + i, k0 + k) : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) {
+        const int idx = tid + 256 * q;
+        const int i = kTrans ? (idx & 63) : (idx >> 6);
+        const int k = kTrans ? (idx >> 6) : (idx & 63);
+        dst[k][i] = v[q];
+    }
+}
+
+// dst[k][j] = X[k0 + k][j] (zero outside krows x cols); kCoherent: agent-coherent loads
+template <typename T, bool kCoherent>
+__device__ __forceinline__ void ts_load_x_tile(T (*dst)[kTsRows + 2], const T* __restrict__ X, long long ldx,
+    long long k0, int krows, int cols, int tid)
+{
+    T v[kTsPer];
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) {
+        const int idx = tid + 256 * q;
+        const int k = idx >> 6, j = idx & 63;
+        const bool in = k < krows && j < cols;
+        if constexpr (kCoherent)
+            v[q] = in ? ld_agent(&X[(k0 + k) * ldx + j]) : T(0);
+        else
+            v[q] = in ? X[(k0 + k) * ldx + j] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) {
+        const int idx = tid + 256 * q;
+        dst[idx >> 6][idx & 63] = v[q];
+    }
+}
+
 // Inverse of every 64 x 64 diagonal block D_b of op(A) (lower triangular when kLower, with an
 // implicit unit diagonal when kUnit; only that triangle is read), stored transposed:
 // dinvT[b * 4096 + j * 64 + i] = inv(D_b)[i][j]. Padding rows and columns of the last block
@@ -158,16 +208,8 @@ __global__ __launch_bounds__(256) void trsm_step_kernel(const T* __restrict__ A,
     if (has_src) {
         const long long k0 = s * kTsRows;
         const int krows = static_cast<int>(min(static_cast<long long>(kTsRows), n - k0));
-        for (int idx = tid; idx < kTsRows * kTsRows; idx += 256) {
-            // op(A)[r0 + i][k0 + k], read along the contiguous index of A
-            const int i = kTrans ? (idx & 63) : (idx >> 6);
-            const int k = kTrans ? (idx >> 6) : (idx & 63);
-            As[k][i] = (i < rows && k < krows) ? op_coef<T, kTrans>(A, lda, r0 + i, k0 + k) : T(0);
-        }
-        for (int idx = tid; idx < kTsRows * kTsRows; idx += 256) {
-            const int k = idx >> 6, j = idx & 63;
-            Bs[k][j] = (k < krows && j < cols) ? X[(k0 + k) * ldx + c0 + j] : T(0);
-        }
+        ts_load_op_tile<T, kTrans>(As, A, lda, r0, rows, k0, krows, tid);
+        ts_load_x_tile<T, false>(Bs, X + c0, ldx, k0, krows, cols, tid);
         __syncthreads();
         mma_pass();
     }
@@ -201,8 +243,17 @@ __global__ __launch_bounds__(256) void trsm_step_kernel(const T* __restrict__ A,
                 const int j = wn * 32 + ni * 16 + (lane & 15);
                 Bs[i][j] = (i < rows && j < cols) ? X[(r0 + i) * ldx + c0 + j] - acc[mi][ni][v] : T(0);
             }
-    const T* inv = dinvT + t * kTsInvElems;
-    for (int idx = tid; idx < kTsRows * kTsRows; idx += 256) As[idx >> 6][idx & 63] = inv[idx];
+    {
+        const T* inv = dinvT + t * kTsInvElems;
+        T inv_v[kTsPer];
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) inv_v[q] = inv[tid + 256 * q];
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) {
+            const int idx = tid + 256 * q;
+            As[idx >> 6][idx & 63] = inv_v[q];
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -219,6 +270,131 @@ __global__ __launch_bounds__(256) void trsm_step_kernel(const T* __restrict__ A,
                 const int j = wn * 32 + ni * 16 + (lane & 15);
                 if (i < rows && j < cols) X[(r0 + i) * ldx + c0 + j] = acc[mi][ni][v];
             }
+}
+
+// The same solve in one launch: workgroup b owns row tile t (t = b forward, nblk - 1 - b
+// backward) and applies the strips it depends on in the order they become final, waiting for
+// each on its flag (bounded), then finishes its own tile with its diagonal-block inverse and
+// raises its flag. op(A)'s tile for the next strip and inv(D_t) are loaded before the wait, so
+// the critical path per strip is one flag hand-off, the strip's 64 x 64 x 64 product and the
+// inverse's. Every workgroup must be resident at once (the host checks the occupancy); a wait
+// that expires raises *err and the caller poisons the result. X_s is read with
+// agent-coherent loads after the flag, and each tile is stored with agent-coherent stores
+// that complete before the flag is raised (no fences: an agent-scope acquire would invalidate
+// the L2 of the reading XCD on every poll).
+template <typename T, bool kLower, bool kTrans>
+__global__ __launch_bounds__(256) void trsm_chain_kernel(const T* __restrict__ A, long long lda, long long n,
+    const T* __restrict__ dinvT, T* __restrict__ X, long long ldx, long long nrhs, unsigned* __restrict__ flags,
+    unsigned spin_limit, int* __restrict__ err, const int* __restrict__ skip)
+{
+    if (skip != nullptr && *skip != 0) return;
+    using MF = Mfma16<T>;
+    using acc_t = typename MF::acc_t;
+    constexpr int PAD = 2;
+    __shared__ T As[kTsRows][kTsRows + PAD];
+    __shared__ T Bs[kTsRows][kTsRows + PAD];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const long long nblk = (n + kTsRows - 1) / kTsRows;
+    const long long t = kLower ? static_cast<long long>(blockIdx.x) : nblk - 1 - blockIdx.x;
+    const long long r0 = t * kTsRows;
+    const int rows = static_cast<int>(min(static_cast<long long>(kTsRows), n - r0));
+    const int cols = static_cast<int>(min(static_cast<long long>(kTsRows), nrhs));
+
+    acc_t acc[2][2];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = acc_t{0, 0, 0, 0};
+
+    auto mma_pass = [&]() {
+#pragma unroll
+        for (int kk = 0; kk < kTsRows; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            T a[2], b[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) a[mi] = As[kr][wm * 32 + mi * 16 + (lane & 15)];
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) b[ni] = Bs[kr][wn * 32 + ni * 16 + (lane & 15)];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = MF::mma(a[mi], b[ni], acc[mi][ni]);
+        }
+    };
+
+    // this tile's right-hand side (in the product's element layout) and its inverse, loaded
+    // before any wait
+    T xt[2][2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wm * 32 + mi * 16 + MF::row(lane, v);
+                const int j = wn * 32 + ni * 16 + (lane & 15);
+                xt[mi][ni][v] = (i < rows && j < cols) ? X[(r0 + i) * ldx + j] : T(0);
+            }
+    T inv_v[kTsPer];
+    {
+        const T* inv = dinvT + t * kTsInvElems;
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) inv_v[q] = inv[tid + 256 * q];
+    }
+
+    const long long deps = kLower ? t : nblk - 1 - t;
+    for (long long q = 0; q < deps; ++q) {
+        const long long sidx = kLower ? q : nblk - 1 - q;
+        const long long k0 = sidx * kTsRows;
+        const int krows = static_cast<int>(min(static_cast<long long>(kTsRows), n - k0));
+        __syncthreads();  // the previous strip's operands are consumed
+        ts_load_op_tile<T, kTrans>(As, A, lda, r0, rows, k0, krows, tid);
+        if (tid == 0) wait_flag_coherent(&flags[sidx], 1u, spin_limit, err);
+        __syncthreads();
+        ts_load_x_tile<T, true>(Bs, X, ldx, k0, krows, cols, tid);
+        __syncthreads();
+        mma_pass();
+    }
+
+    // own tile: Y = X_t - acc, then X_t = inv(D_t) Y
+    __syncthreads();
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wm * 32 + mi * 16 + MF::row(lane, v);
+                const int j = wn * 32 + ni * 16 + (lane & 15);
+                Bs[i][j] = (i < rows && j < cols) ? xt[mi][ni][v] - acc[mi][ni][v] : T(0);
+            }
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) {
+        const int idx = tid + 256 * q;
+        As[idx >> 6][idx & 63] = inv_v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = acc_t{0, 0, 0, 0};
+    mma_pass();
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wm * 32 + mi * 16 + MF::row(lane, v);
+                const int j = wn * 32 + ni * 16 + (lane & 15);
+                if (i < rows && j < cols) st_agent(&X[(r0 + i) * ldx + j], static_cast<T>(acc[mi][ni][v]));
+            }
+    wait_own_stores();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace lrn

@@ -23,6 +23,15 @@ template <typename T>
 int trsm_steps_launch(bool lower, bool trans, const T* a, long long lda, long long n, const T* dinv, T* x,
     long long ldx, long long nrhs, const int* skip, hipStream_t s);
 
+// The same solve as trsm_steps_launch in one launch per 64 columns of X (trsm_chain_kernel):
+// workgroups chained through flags[] (trsm_flag_count(n, nrhs) zeroed words), bounded waits
+// raising *err. Returns LRN_E_UNSUPPORTED, launching nothing, when the tiles' workgroups
+// cannot all be resident at once.
+long long trsm_flag_count(long long n, long long nrhs);
+template <typename T>
+int trsm_chain_launch(bool lower, bool trans, const T* a, long long lda, long long n, const T* dinv, T* x,
+    long long ldx, long long nrhs, unsigned* flags, unsigned spin_limit, int* err, const int* skip, hipStream_t s);
+
 // Y[i][c] = B[perm[i]][c] (B[i][c] when perm is null) for i < n, c < nc.
 template <typename T>
 int gather_rows_launch(const T* b, long long ldb, const int* perm, T* y, long long ldy, long long n, long long nc,

@@ -2,7 +2,10 @@
 at 0 every wait gives up at once (each kernel still runs to its end with in-range indices),
 raises the error word, and the results must say so. The synchronous calls report
 LRN_E_TIMEOUT; the asynchronous solvers leave NaN in x instead of a wrong solution; the
-inversion does not copy its (invalid) result over m. The limit is restored after each test."""
+inversion does not copy its (invalid) result over m. The limit is restored after each test.
+The many-right-hand-side solves run in each of their three forms: the chained substitution
+and the blocked form's chained kernel wait between workgroups (NaN expected); the blocked form
+with one launch per strip has no such wait and must solve correctly under the same limit."""
 import numpy as np
 import pytest
 
@@ -48,9 +51,26 @@ def test_solver_lu_mode_leaves_nan(expired_waits, nrhs, dev):
     assert np.isnan(xs).all()
 
 
-def test_solver_upfront_substitution_leaves_nan(dev):
+# many-right-hand-side kernels: (lrn_set_multi_rhs_solver, lrn_set_multi_rhs_kernel, waits between workgroups)
+MULTI_RHS = {"chained": (0, 1, True), "blocked_chain": (2, 1, True), "blocked_steps": (2, 0, False)}
+
+
+def _multi_rhs(lib, mode):
+    min_rhs, kind, _ = MULTI_RHS[mode]
+    assert lib.lrn_set_multi_rhs_solver(min_rhs) == 0
+    assert lib.lrn_set_multi_rhs_kernel(kind) == 0
+
+
+def _multi_rhs_default(lib):
+    assert lib.lrn_set_multi_rhs_solver(2) == 0
+    assert lib.lrn_set_multi_rhs_kernel(1) == 0
+
+
+@pytest.mark.parametrize("mode", list(MULTI_RHS))
+def test_solver_upfront_substitution_leaves_nan(mode, dev):
     """Valid factors (computed with the normal limit), then a solve whose substitution waits
-    expire: x is NaN."""
+    expire: x is NaN, for one and five right-hand sides; the blocked form with one launch per
+    strip, which has no wait between workgroups, solves five correctly under the same limit."""
     import torch
     import linearalgebra_amd as la
     from linearalgebra_amd import _lib
@@ -64,22 +84,38 @@ def test_solver_upfront_substitution_leaves_nan(dev):
         b = O.gaussian(n, nrhs, seed=5)
         x = torch.from_numpy(b[:, 0].copy() if nrhs == 1 else b).to(dev)
         assert lib.lrn_debug_set_spin_limit(0) == 0
+        _multi_rhs(lib, mode)
         try:
             la.linear_solver.apply(m, p, n, x, la.linear_solver_type.LU_UPFRONT_LINEAR_SOLVER)
             torch.cuda.synchronize()
         finally:
             assert lib.lrn_debug_set_spin_limit(-1) == 0
-        assert np.isnan(x.cpu().numpy()).all()
+            _multi_rhs_default(lib)
+        ref = O.getrs(lu, piv, b[:, 0] if nrhs == 1 else b)
+        if nrhs > 1 and not MULTI_RHS[mode][2]:
+            assert O.forward_error(x.cpu().numpy(), ref) <= 1e-12
+        else:
+            assert np.isnan(x.cpu().numpy()).all()
         # and the same call with the normal limit solves correctly
         x = torch.from_numpy(b[:, 0].copy() if nrhs == 1 else b).to(dev)
-        la.linear_solver.apply(m, p, n, x, la.linear_solver_type.LU_UPFRONT_LINEAR_SOLVER)
-        torch.cuda.synchronize()
-        assert O.forward_error(x.cpu().numpy(), O.getrs(lu, piv, b[:, 0] if nrhs == 1 else b)) <= 1e-12
+        _multi_rhs(lib, mode)
+        try:
+            la.linear_solver.apply(m, p, n, x, la.linear_solver_type.LU_UPFRONT_LINEAR_SOLVER)
+            torch.cuda.synchronize()
+        finally:
+            _multi_rhs_default(lib)
+        assert O.forward_error(x.cpu().numpy(), ref) <= 1e-12
 
 
-def test_triangular_and_cholesky_solves_leave_nan(expired_waits, dev):
+@pytest.mark.parametrize("mode", list(MULTI_RHS))
+def test_triangular_and_cholesky_solves_leave_nan(expired_waits, mode, dev):
+    """Two right-hand sides under the forced expiry: NaN where the kernel waits between
+    workgroups; the blocked form with one launch per strip solves the triangle correctly."""
     import torch
     import linearalgebra_amd as la
+    from oracle import oracle as OM
+    lib = expired_waits
+    waits = MULTI_RHS[mode][2]
     n = 300
     rng = np.random.default_rng(6)
     g = rng.standard_normal((n, n))
@@ -87,14 +123,22 @@ def test_triangular_and_cholesky_solves_leave_nan(expired_waits, dev):
     b = rng.standard_normal((n, 2))
     a_t = torch.from_numpy(np.tril(spd)).to(dev)
     x = torch.zeros(n, 2, dtype=torch.float64, device=dev)
-    la.triangular_solver.apply(False, False, False, n, 2, a_t, n, torch.from_numpy(b).to(dev), 2, x, 2)
-    torch.cuda.synchronize()
-    assert np.isnan(x.cpu().numpy()).all()
-    m = torch.from_numpy(spd).to(dev)
-    xc = torch.from_numpy(b.copy()).to(dev)
-    la.linear_solver.apply(m, None, n, xc, la.linear_solver_type.CHOLESKY_LINEAR_SOLVER)
-    torch.cuda.synchronize()
-    assert np.isnan(xc.cpu().numpy()).all()
+    _multi_rhs(lib, mode)
+    try:
+        la.triangular_solver.apply(False, False, False, n, 2, a_t, n, torch.from_numpy(b).to(dev), 2, x, 2)
+        torch.cuda.synchronize()
+        if waits:
+            assert np.isnan(x.cpu().numpy()).all()
+        else:
+            assert OM.rel_diff(x.cpu().numpy(), OM.trsm(False, False, False, np.tril(spd), b)) <= 1e-12
+        m = torch.from_numpy(spd).to(dev)
+        xc = torch.from_numpy(b.copy()).to(dev)
+        la.linear_solver.apply(m, None, n, xc, la.linear_solver_type.CHOLESKY_LINEAR_SOLVER)
+        torch.cuda.synchronize()
+    finally:
+        _multi_rhs_default(lib)
+    if waits:
+        assert np.isnan(xc.cpu().numpy()).all()
 
 
 def test_inversion_and_mixed_report_timeout(expired_waits, dev):

@@ -293,14 +293,20 @@ def test_solver_multi_rhs_blocked_matches_chained(n, nrhs, dev):
     from linearalgebra_amd import _lib
     a = O.diag_dominant(n, seed=40 + n)
     b = O.gaussian(n, nrhs, seed=41 + n)
+    lib = _lib.load()
     try:
         _, piv_b, x_b = _gpu_solve(a, b, dev, mode=1)
-        assert _lib.load().lrn_set_multi_rhs_solver(0) == 0
+        assert lib.lrn_set_multi_rhs_kernel(0) == 0  # one launch per strip
+        _, piv_s, x_s = _gpu_solve(a, b, dev, mode=1)
+        assert lib.lrn_set_multi_rhs_kernel(1) == 0
+        assert lib.lrn_set_multi_rhs_solver(0) == 0
         _, piv_c, x_c = _gpu_solve(a, b, dev, mode=1)
     finally:
-        _lib.load().lrn_set_multi_rhs_solver(2)
+        lib.lrn_set_multi_rhs_kernel(1)
+        lib.lrn_set_multi_rhs_solver(2)
     lu_o, piv_o, x_o, info_o = O.gesv(a, b, threads=16)
-    d_bc, d_o = O.forward_error(x_b, x_c), O.forward_error(x_b, x_o)
-    print(f"multi-rhs n={n} nrhs={nrhs}: blocked vs chained {d_bc:.2e}, vs oracle {d_o:.2e}", flush=True)
-    assert np.array_equal(piv_b, piv_o) and np.array_equal(piv_c, piv_o)
-    assert d_bc <= 1e-12 and d_o <= 1e-12
+    d_bc, d_sc, d_o = O.forward_error(x_b, x_c), O.forward_error(x_s, x_c), O.forward_error(x_b, x_o)
+    print(f"multi-rhs n={n} nrhs={nrhs}: blocked vs chained {d_bc:.2e} (one launch per strip {d_sc:.2e}), "
+          f"vs oracle {d_o:.2e}", flush=True)
+    assert np.array_equal(piv_b, piv_o) and np.array_equal(piv_c, piv_o) and np.array_equal(piv_s, piv_o)
+    assert d_bc <= 1e-12 and d_sc <= 1e-12 and d_o <= 1e-12

@@ -74,9 +74,10 @@ def _multi_rhs_solver(min_rhs):
 @pytest.mark.parametrize("upper,transpose,unit", [(False, False, True), (True, False, False), (False, True, False),
                                                   (True, True, True)])
 def test_triangular_blocked_matches_chained_and_oracle(n, nrhs, upper, transpose, unit, dev):
-    """The blocked form (inverted 64 x 64 diagonal blocks, one matrix-core update per strip;
-    nrhs > 64 covers several column groups, n = 4100 a partial last strip) against the chained
-    substitution and the oracle, on a well-conditioned triangle: both within 1e-12."""
+    """The blocked form (inverted 64 x 64 diagonal blocks, matrix-core updates; its chained
+    single-launch kernel and its launch-per-strip kernel; nrhs > 64 covers several column groups,
+    n = 4100 a partial last strip) against the chained substitution and the oracle, on a
+    well-conditioned triangle: all within 1e-12."""
     rng = np.random.default_rng(3 * n + nrhs)
     a = 0.2 * rng.standard_normal((n, n)) / np.sqrt(n) + np.diag(2.0 + rng.random(n))
     # NaN outside the triangle op(A) uses: neither form may read it
@@ -84,17 +85,26 @@ def test_triangular_blocked_matches_chained_and_oracle(n, nrhs, upper, transpose
     mask = np.tril(np.ones((n, n), dtype=bool)) if lower_stored else np.triu(np.ones((n, n), dtype=bool))
     a_nan = np.where(mask, a, np.nan)
     b = rng.standard_normal((n, nrhs))
+    from linearalgebra_amd import _lib
+    lib = _lib.load()
     try:
         _, _, xb = _solve(upper, transpose, unit, n, nrhs, a_nan, n, b, nrhs, nrhs + 3, dev)
+        assert lib.lrn_set_multi_rhs_kernel(0) == 0  # one launch per strip
+        _, _, xs = _solve(upper, transpose, unit, n, nrhs, a_nan, n, b, nrhs, nrhs + 3, dev)
+        assert lib.lrn_set_multi_rhs_kernel(1) == 0
         _multi_rhs_solver(0)
         _, _, xc = _solve(upper, transpose, unit, n, nrhs, a_nan, n, b, nrhs, nrhs + 3, dev)
     finally:
+        lib.lrn_set_multi_rhs_kernel(1)
         _multi_rhs_solver(2)
-    assert (xb[:, nrhs:] == -123).all() and (xc[:, nrhs:] == -123).all()
-    xb, xc = xb[:, :nrhs], xc[:, :nrhs]
-    assert np.isfinite(xb).all()
+    assert (xb[:, nrhs:] == -123).all() and (xc[:, nrhs:] == -123).all() and (xs[:, nrhs:] == -123).all()
+    xb, xc, xs = xb[:, :nrhs], xc[:, :nrhs], xs[:, :nrhs]
+    assert np.isfinite(xb).all() and np.isfinite(xs).all()
     d_bc = np.abs(xb - xc).max() / np.abs(xc).max()
-    msg = f"trsm blocked n={n} nrhs={nrhs} upper={upper} transpose={transpose} unit={unit}: vs chained {d_bc:.2e}"
+    d_sc = np.abs(xs - xc).max() / np.abs(xc).max()
+    assert d_sc <= 1e-12
+    msg = (f"trsm blocked n={n} nrhs={nrhs} upper={upper} transpose={transpose} unit={unit}: vs chained {d_bc:.2e} "
+           f"(one launch per strip: {d_sc:.2e})")
     if n <= 1000:
         ref = O.trsm(upper, transpose, unit, a, b)
         d_o = np.abs(xb - ref).max() / np.abs(ref).max()
