@@ -434,10 +434,30 @@ int launch_laswp_trsm(T* A, long long lda, const int* rowmap, int W2, long long 
     return LRN_OK;
 }
 
-// C[M x N] -= A[M x K] * B[K x N]
+// Algorithmic flops of one trailing-update launch: 2 M N K, or in the lower-triangle mode
+// (tri != kNoTriangle) those of the tiles it computes.
+double gemm_flops(long long m, long long nn, long long K, long long tri, int bm, int bn)
+{
+    if (tri == kNoTriangle) return 2.0 * m * nn * K;
+    double f = 0.0;
+    for (long long r0 = 0; r0 < m; r0 += bm) {
+        const long long rows = std::min<long long>(bm, m - r0);
+        // columns c with c + tri <= r0 + bm - 1, in whole tiles
+        const long long last = r0 + bm - 1 - tri;  // largest local column a computed tile may start at
+        if (last < 0) continue;
+        const long long ntiles = std::min<long long>((last / bn) + 1, (nn + bn - 1) / bn);
+        const long long cols = std::min<long long>(ntiles * bn, nn);
+        f += 2.0 * rows * cols * K;
+    }
+    return f;
+}
+
+// C[M x N] -= A[M x K] * B[K x N]. lower: C is a square block on the diagonal of a symmetric
+// update of which only the lower triangle is read later; tiles entirely above the diagonal
+// are not computed (a tile straddling it is computed whole).
 template <typename T>
 int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, long long M,
-    long long N, long long K, hipStream_t s, int cus = 0, const int* skip = nullptr)
+    long long N, long long K, hipStream_t s, int cus = 0, const int* skip = nullptr, bool lower = false)
 {
     if (M <= 0 || N <= 0 || K <= 0) return LRN_OK;
     if constexpr (std::is_same<T, double>::value) {
@@ -446,8 +466,10 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
                              (lda % 2 == 0) && (ldb % 2 == 0) && (K % kG64BK == 0);
         const long long Mi = aligned ? M / kG64BM * kG64BM : 0;
         const long long Ni = aligned ? N / kG64BN * kG64BN : 0;
-        auto launch = [&](bool check, const double* a, const double* b, double* c, long long m, long long nn) -> int {
+        auto launch = [&](bool check, const double* a, const double* b, double* c, long long m, long long nn,
+                          long long coff) -> int {
             if (m <= 0 || nn <= 0) return LRN_OK;
+            const long long tri = lower ? coff : kNoTriangle;
             const long long tm = (m + kG64BM - 1) / kG64BM, tn = (nn + kG64BN - 1) / kG64BN;
             if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
             const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
@@ -456,21 +478,21 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             LRN_HIP(hipGetDevice(&dev));
             const long long cap = 2ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
-            TraceScope ts(check ? kTrGemmF64Edge : kTrGemmF64, 2.0 * m * nn * K, s);
+            TraceScope ts(check ? kTrGemmF64Edge : kTrGemmF64, gemm_flops(m, nn, K, tri, kG64BM, kG64BN), s);
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<true>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip, tri);
             else
                 hipLaunchKernelGGL(gemm_sub_f64_kernel<false>, grid, dim3(512), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip, tri);
             LRN_HIP(hipGetLastError());
             return LRN_OK;
         };
-        LRN_TRY(launch(false, A, B, C, Mi, Ni));
-        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni));                       // right strip, all rows
-        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni));          // bottom strip
+        LRN_TRY(launch(false, A, B, C, Mi, Ni, 0));
+        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni, Ni));                   // right strip, all rows
+        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni, -Mi));     // bottom strip
         return LRN_OK;
     } else {
         // FP32: the same interior / ragged-strip split on the double-buffered kernel
@@ -478,8 +500,10 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
                              (lda % 4 == 0) && (ldb % 4 == 0) && (K % kG32BK == 0);
         const long long Mi = aligned ? M / kG32BM * kG32BM : 0;
         const long long Ni = aligned ? N / kG32BN * kG32BN : 0;
-        auto launch = [&](bool check, const float* a, const float* b, float* c, long long m, long long nn) -> int {
+        auto launch = [&](bool check, const float* a, const float* b, float* c, long long m, long long nn,
+                          long long coff) -> int {
             if (m <= 0 || nn <= 0) return LRN_OK;
+            const long long tri = lower ? coff : kNoTriangle;
             const long long tm = (m + kG32BM - 1) / kG32BM, tn = (nn + kG32BN - 1) / kG32BN;
             if (tm * tn > 0x3fffffffll) return LRN_E_UNSUPPORTED;
             const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
@@ -487,21 +511,21 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             LRN_HIP(hipGetDevice(&dev));
             const long long cap = 2ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
-            TraceScope ts(kTrGemmF32, 2.0 * m * nn * K, s);
+            TraceScope ts(kTrGemmF32, gemm_flops(m, nn, K, tri, kG32BM, kG32BN), s);
             if (check)
                 hipLaunchKernelGGL(gemm_sub_f32_kernel<true>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip, tri);
             else
                 hipLaunchKernelGGL(gemm_sub_f32_kernel<false>, grid, dim3(256), 0, s, a, lda, b, ldb, c, ldc,
                     static_cast<int>(m), static_cast<int>(nn), static_cast<int>(K), static_cast<int>(tm),
-                    static_cast<int>(tn), static_cast<int>(per_xcd), skip);
+                    static_cast<int>(tn), static_cast<int>(per_xcd), skip, tri);
             LRN_HIP(hipGetLastError());
             return LRN_OK;
         };
-        LRN_TRY(launch(false, A, B, C, Mi, Ni));
-        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni));
-        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni));
+        LRN_TRY(launch(false, A, B, C, Mi, Ni, 0));
+        LRN_TRY(launch(true, A, B + Ni, C + Ni, M, N - Ni, Ni));
+        LRN_TRY(launch(true, A + Mi * lda, B, C + Mi * ldc, M - Mi, Ni, -Mi));
         return LRN_OK;
     }
 }
@@ -633,10 +657,39 @@ struct AsideStream {
     std::mutex mu;
 };
 
+// The helper streams and events live until the process ends; they are released by an exit
+// handler registered after the HIP runtime has started (so it runs before the runtime's own
+// teardown) rather than left to the runtime's teardown. The table itself is never destroyed.
+std::mutex g_aside_mu;
+std::vector<std::vector<std::unique_ptr<AsideStream>>>& aside_table()
+{
+    static auto* t = new std::vector<std::vector<std::unique_ptr<AsideStream>>>();
+    return *t;
+}
+
+void release_aside_streams()
+{
+    std::lock_guard<std::mutex> g(g_aside_mu);
+    for (auto& dev_list : aside_table())
+        for (auto& a : dev_list) {
+            (void)hipStreamSynchronize(a->narrow);
+            (void)hipStreamSynchronize(a->panel);
+            (void)hipStreamSynchronize(a->gemm);
+            (void)hipEventDestroy(a->start);
+            (void)hipEventDestroy(a->ready);
+            (void)hipEventDestroy(a->done);
+            (void)hipEventDestroy(a->rest_done);
+            (void)hipStreamDestroy(a->narrow);
+            (void)hipStreamDestroy(a->panel);
+            (void)hipStreamDestroy(a->gemm);
+        }
+    aside_table().clear();
+}
+
 int aside_stream(int np, AsideStream** out)
 {
-    static std::mutex mu;
-    static std::vector<std::vector<std::unique_ptr<AsideStream>>> per_dev;
+    static std::mutex& mu = g_aside_mu;
+    auto& per_dev = aside_table();
     int dev = 0;
     LRN_HIP(hipGetDevice(&dev));
     const int cus = device_props(dev).cus;
@@ -662,6 +715,8 @@ int aside_stream(int np, AsideStream** out)
     LRN_HIP(hipEventCreateWithFlags(&a->rest_done, hipEventDisableTiming));
     *out = a.get();
     per_dev[dev].push_back(std::move(a));
+    static const bool at_exit_registered = (std::atexit(release_aside_streams) == 0);
+    (void)at_exit_registered;
     return LRN_OK;
 }
 
@@ -1191,7 +1246,7 @@ template <typename T> int chol_factor(T* A, long long n, int upper, const CholLa
         LRN_HIP(hipGetLastError());
         // skipped once a diagonal block has failed (the reference stops at the failing block)
         LRN_TRY(launch_gemm_sub<T>(A + (k + bs) * n + k, n, L.wt, ldw, A + (k + bs) * n + (k + bs), n, m, m, bs, s, 0,
-            L.info));
+            L.info, true));
     }
     const long long pairs = n * (n - 1) / 2;
     const unsigned grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>((pairs + 255) / 256, 1ll << 16)));

@@ -70,7 +70,7 @@ __device__ __forceinline__ void g32_load(g32v4 (&ra)[4], g32v4 (&rb)[4], const f
 template <bool kCheck>
 __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(const float* __restrict__ A, long long lda,
     const float* __restrict__ B, long long ldb, float* __restrict__ C, long long ldc, int M, int N, int K, int tm,
-    int tn, int per_xcd, const int* __restrict__ skip)
+    int tn, int per_xcd, const int* __restrict__ skip, long long tri)
 {
     if (skip != nullptr && *skip != 0) return;  // e.g. a Cholesky factorisation that has failed
     constexpr int BM = kG32BM, BN = kG32BN, BK = kG32BK;
@@ -83,6 +83,11 @@ __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(const float* __restri
     for (int vt = blockIdx.x; vt < nvirt; vt += gridDim.x) {
         int tile_m, tile_n;
         if (!xcd_tile(vt, tm, tn, per_xcd, tile_m, tile_n)) continue;
+        // lower-triangle mode (tri != kNoTriangle): a tile whose columns all lie right of its
+        // rows (global column = local column + tri, relative to the same diagonal) is skipped
+        if (tri != kNoTriangle &&
+            static_cast<long long>(tile_n) * kG32BN + tri > static_cast<long long>(tile_m) * kG32BM + kG32BM - 1)
+            continue;
         const int tid = threadIdx.x;
         const int lane = tid & 63;
         const int wid = tid >> 6;

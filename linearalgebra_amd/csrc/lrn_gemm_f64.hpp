@@ -26,6 +26,8 @@
 namespace lrn {
 
 constexpr int kG64BM = 128, kG64BN = 128, kG64BK = 16;
+// tri argument of the trailing-update GEMMs that disables the lower-triangle mode
+constexpr long long kNoTriangle = -(1ll << 62);
 typedef double g64v2 __attribute__((ext_vector_type(2)));  // native vector: stays in registers
 constexpr int kXcds = 8;
 constexpr int kGroupRows = 8;
@@ -107,7 +109,7 @@ __device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const d
 template <bool kCheck>
 __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __restrict__ A, long long lda,
     const double* __restrict__ B, long long ldb, double* __restrict__ C, long long ldc, int M, int N, int K, int tm,
-    int tn, int per_xcd, const int* __restrict__ skip)
+    int tn, int per_xcd, const int* __restrict__ skip, long long tri)
 {
     if (skip != nullptr && *skip != 0) return;  // e.g. a Cholesky factorisation that has failed
     constexpr int BM = kG64BM, BN = kG64BN, BK = kG64BK;
@@ -122,6 +124,11 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
     for (int vt = blockIdx.x; vt < nvirt; vt += gridDim.x) {
     int tile_m, tile_n;
     if (!xcd_tile(vt, tm, tn, per_xcd, tile_m, tile_n)) continue;
+    // lower-triangle mode (tri != kNoTriangle): a tile whose columns all lie right of its rows
+    // (global column = local column + tri, relative to the same diagonal) is skipped
+    if (tri != kNoTriangle &&
+        static_cast<long long>(tile_n) * kG64BN + tri > static_cast<long long>(tile_m) * kG64BM + kG64BM - 1)
+        continue;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wid = tid >> 6;

@@ -697,24 +697,47 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
 #pragma unroll
             for (int e = 0; e < CPT; ++e) U[32 * p + r][cg + e] = x[e];
         }
-        for (int idx = tid; idx < 32 * 32; idx += 256) {
-            const int i = idx >> 5, q = idx & 31;
-            Ld[i][q] = (i < rp && q < rp) ? A[(r0 + i) * lda + r0 + q] : T(0);
+        {
+            // the thread's four loads in flight together
+            T dv[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int idx = tid + 256 * e;
+                const int i = idx >> 5, q = idx & 31;
+                dv[e] = (i < rp && q < rp) ? A[(r0 + i) * lda + r0 + q] : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int idx = tid + 256 * e;
+                Ld[idx >> 5][idx & 31] = dv[e];
+            }
         }
         __syncthreads();
         if (tid < TN) {
-            // the column stays in LDS; the diagonal-block entries are broadcast reads
+            // the column's 32 entries in registers (fully unrolled, so every index is a
+            // constant); the diagonal-block entries are broadcast reads that do not depend on
+            // the chain, so only the products and subtractions are on it
             const int c = tid;
-            T* ucol = &U[32 * p][c];
-#pragma unroll 1
-            for (int i = 1; i < rp; ++i) {
-                T sum = ucol[i * (TN + 1)];
-#pragma unroll 8
-                for (int q = 0; q < i; ++q) sum = sum - Ld[i][q] * ucol[q * (TN + 1)];
-                ucol[i * (TN + 1)] = sum;
+            T u[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) u[i] = (i < rp) ? U[32 * p + i][c] : T(0);
+#pragma unroll
+            for (int i = 1; i < 32; ++i) {
+                if (i < rp) {
+                    T sum = u[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) sum = sum - Ld[i][q] * u[q];
+                    u[i] = sum;
+                }
             }
-            if (c < ncols)
-                for (int i = 0; i < rp; ++i) A[(r0 + i) * lda + cb + c] = ucol[i * (TN + 1)];
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (i < rp) U[32 * p + i][c] = u[i];
+            if (c < ncols) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i)
+                    if (i < rp) A[(r0 + i) * lda + cb + c] = u[i];
+            }
         }
         __syncthreads();
     }

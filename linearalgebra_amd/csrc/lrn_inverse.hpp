@@ -24,18 +24,21 @@ __global__ __launch_bounds__(256) void lu_det_kernel(const T* __restrict__ LU, l
     if (t == 0) odd = 0;
     T prod = T(1);
     int mine = 0;
+    // the next 256 diagonal entries are loaded while lane 0 multiplies the current ones
+    T cur = (t < n) ? LU[static_cast<long long>(t) * lda + t] : T(1);
     for (long long i0 = 0; i0 < n; i0 += 256) {
         const long long i = i0 + t;
+        const long long inext = i + 256;
+        const T nxt = (inext < n) ? LU[inext * lda + inext] : T(1);
+        if (i < n && piv != nullptr && piv[i] != i + 1) mine ^= 1;
         __syncthreads();
-        if (i < n) {
-            diag[t] = LU[i * lda + i];
-            if (piv != nullptr && piv[i] != i + 1) mine ^= 1;
-        }
+        diag[t] = cur;
         __syncthreads();
         if (t == 0) {
             const int cnt = static_cast<int>(n - i0 < 256 ? n - i0 : 256);
             for (int k = 0; k < cnt; ++k) prod *= diag[k];
         }
+        cur = nxt;
     }
     if (mine) atomicXor(&odd, 1);
     __syncthreads();

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -76,11 +77,26 @@ inline void trace_end(long idx, hipStream_t s)
     (void)hipEventRecord(t.recs[static_cast<size_t>(idx)].b, s);
 }
 
+// The event pool is released by an exit handler registered after the HIP runtime has
+// started, so it runs before the runtime's own teardown.
+inline void trace_release()
+{
+    Tracer& t = tracer();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.mask.store(0);
+    for (hipEvent_t e : t.pool) (void)hipEventDestroy(e);
+    t.pool.clear();
+    t.recs.clear();
+    t.used = 0;
+}
+
 // Starts a trace of the kinds in `mask` with room for `capacity` launches (events are
 // created here, outside any timed region).
 inline int trace_start(unsigned mask, long capacity)
 {
-    Tracer& t = tracer();
+    Tracer& t = tracer();  // constructed first, so the exit handler runs before its destructor
+    static const bool at_exit_registered = (std::atexit(trace_release) == 0);
+    (void)at_exit_registered;
     std::lock_guard<std::mutex> g(t.mu);
     t.mask.store(0);
     const size_t need = static_cast<size_t>(capacity) * 2;
