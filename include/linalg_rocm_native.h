@@ -260,9 +260,14 @@ LRN_API int lrn_set_mixed_update(int mode);
  * path (the default). Either way the factors only precondition the FP64 refinement. */
 LRN_API int lrn_set_mixed_panel(int tournament);
 /* Triangular solves of the mixed solver's refinement (FP32 factors applied in FP64,
- * process-wide, no reference counterpart): 0 = the 64-row kernel of the plain single-RHS
- * solve (the default); 1 = 256-row, 2 = 128-row workgroups chained through tagged hand-offs,
- * each solving its rows in 64-row sub-blocks. */
+ * process-wide, no reference counterpart): 0 = the 64-row substitution kernel of the plain
+ * single-RHS solve; 1 = 256-row, 2 = 128-row workgroups chained through tagged hand-offs,
+ * each solving its rows in 64-row sub-blocks; 3 = 64-row blocks in the inverse form: block b
+ * is x_b = z_b - M1_b x_(b-+1) - M2_b x_(b-+2) with z_b = inv(D_b) (rhs_b - the blocks before),
+ * M_b = inv(D_b) A[b][b-+1 or 2], all computed in FP64 from the FP32 factors once per
+ * factorisation, so the step after each hand-off is a short dot product instead of a 64-step
+ * substitution; 4 = mode 3 with runs of consecutive blocks placed on one XCD when all of its
+ * workgroups can be resident at once (the default). */
 LRN_API int lrn_set_refine_solver(int chain);
 /* Kernel of the 32-column cooperative panel (process-wide, no reference counterpart; same
  * pivots and factors either way): 0 = panel rows held in registers, one row per thread of

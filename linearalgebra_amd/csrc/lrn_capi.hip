@@ -964,7 +964,7 @@ template <typename T> size_t solver_ws_bytes(long long n, long long nrhs, int ty
 // Single-RHS substitution with FP32 factors in FP64 arithmetic (the refinement's
 // preconditioner): 0 = the 64-row kernel (the default), 1 = 256-row chained kernel (streams
 // its factor rows through too few compute units to keep up), 2 = 128-row chained kernel.
-std::atomic<int> g_chain_solve{0};
+std::atomic<int> g_chain_solve{4};  // refinement substitution (lrn_set_refine_solver)
 // Refinement of the mixed solver: 0 = LU-IR while each step gains at least 2x, then GMRES-IR;
 // 1 = GMRES-IR right after the first LU-IR step.
 std::atomic<int> g_refinement{0};
@@ -1059,7 +1059,35 @@ int getrs_subst(const TF* LU, long long n, T* B, long long nrhs, long long ldb, 
             TraceScope ts(kTrTrsv, 2.0 * tri_bytes, s);
             if constexpr (std::is_same<TF, float>::value && std::is_same<T, double>::value) {
                 const int chain = g_chain_solve.load();
-                if (chain != 0 && ldb == 1) {
+                if ((chain == 3 || chain == 4) && L.dinv != nullptr) {
+                    // the diagonal blocks applied through their inverses and the predecessor
+                    // through its precomputed product (in L.dinv, written by the caller with
+                    // refine_inverses); mode 4 also keeps runs of consecutive blocks on one XCD
+                    // when every workgroup can be resident
+                    int xcd_per = 0;
+                    if (chain == 4 && nblk >= 2 * kXcds) {
+                        int per_cu = 0, dev = 0;
+                        LRN_TRY(prepare_kernel(reinterpret_cast<const void*>(trsv1_inv_kernel<float, double, true>), 0,
+                            256, &per_cu));
+                        LRN_HIP(hipGetDevice(&dev));
+                        if (nblk <= static_cast<long long>(per_cu) * device_props(dev).cus)
+                            xcd_per = static_cast<int>((nblk + kXcds - 1) / kXcds);
+                    }
+                    const unsigned grid = static_cast<unsigned>(xcd_per > 0 ? kXcds * xcd_per : nblk);
+                    const long long e = trsm_dinv_elems(n);
+                    const double* dl = L.dinv;
+                    const double* du = L.dinv + e;
+                    hipLaunchKernelGGL((trsv1_inv_kernel<float, double, true>), dim3(grid), dim3(256), 0, s, LU, n, n,
+                        L.perm, B + c0, ldb, L.y, 1ll, L.xch, 1u, spin_limit(), L.err, skip_dev, dl, dl + 2 * e,
+                        dl + 4 * e, xcd_per);
+                    LRN_HIP(hipGetLastError());
+                    hipLaunchKernelGGL((trsv1_inv_kernel<float, double, false>), dim3(grid), dim3(256), 0, s, LU, n,
+                        n, L.perm, L.y, 1ll, B + c0, ldb, L.xch, 2u, spin_limit(), L.err, skip_dev, du, du + 2 * e,
+                        du + 4 * e, xcd_per);
+                    LRN_HIP(hipGetLastError());
+                    continue;
+                }
+                if ((chain == 1 || chain == 2) && ldb == 1) {
                     const bool vec4 = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(LU) % 16 == 0);
                     auto go = [&](auto ns) -> int {
                         constexpr int NS = decltype(ns)::value;
@@ -1700,6 +1728,44 @@ int debug_panel_impl(T* a, int64_t lda, int64_t k, int w, int64_t* pivot, void* 
 
 // ------------------------------------------------------------------- mixed-precision solve
 
+// For the refinement's substitution modes 3 and 4 (lrn_set_refine_solver), from FP32 factors,
+// in FP64: the inverted 64 x 64 diagonal blocks of unit L and of U, then their products with
+// the off-diagonal blocks one and two blocks away (trsv_pred_kernel), in G.dinv
+// (6 x trsm_dinv_elems(n), interleaved: L, U, L dist 1, U dist 1, L dist 2, U dist 2); nothing
+// to do for the other modes. The factors are fixed during the refinement, so this runs
+// once per factorisation.
+int refine_inverses(const float* lu, long long n, const GetrsLayout<double>& G, hipStream_t s)
+{
+    const int chain = g_chain_solve.load();
+    if ((chain != 3 && chain != 4) || G.dinv == nullptr) return LRN_OK;
+    const long long e = trsm_dinv_elems(n);
+    double* const dl = G.dinv;
+    double* const du = G.dinv + e;
+    LRN_TRY(trsm_inv_widen_launch(true, true, lu, n, n, dl, nullptr, s));
+    LRN_TRY(trsm_inv_widen_launch(false, false, lu, n, n, du, nullptr, s));
+    for (int dist = 1; dist <= 2; ++dist) {
+        LRN_TRY(trsv_pred_widen_launch(true, dist, lu, n, n, dl, dl + 2 * dist * e, nullptr, s));
+        LRN_TRY(trsv_pred_widen_launch(false, dist, lu, n, n, du, du + 2 * dist * e, nullptr, s));
+    }
+    return LRN_OK;
+}
+
+// Workspace of the stand-alone preconditioner (lrn_getrs_f32f64): the single-RHS buffers and
+// room for the inverted diagonal blocks.
+GetrsLayout<double> carve_f32f64(Arena& ar, long long n)
+{
+    auto L = carve_getrs<double>(ar, n, 1);
+    L.dinv = ar.next<double>(static_cast<size_t>(6 * trsm_dinv_elems(n)));  // see refine_inverses
+    return L;
+}
+
+size_t f32f64_ws_bytes(long long n)
+{
+    Arena ar;
+    carve_f32f64(ar, n);
+    return ar.off;
+}
+
 struct MixedLayout {
     float* a32;
     long long* piv;
@@ -1735,6 +1801,7 @@ MixedLayout carve_mixed(Arena& ar, long long n, int m, bool fallback)
     L.calu.info = L.f.ws.info;
     L.calu.rowmap = nullptr;
     L.g = carve_getrs<double>(ar, n, 1);
+    L.g.dinv = ar.next<double>(static_cast<size_t>(6 * trsm_dinv_elems(n)));  // see refine_inverses
     L.r = ar.next<double>(static_cast<size_t>(n));
     L.w = ar.next<double>(static_cast<size_t>(n));
     L.d = ar.next<double>(static_cast<size_t>(n));
@@ -1946,6 +2013,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     LRN_HIP(hipGetLastError());
     LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd, g_mixed_panel.load() ? &L.calu : nullptr)));
     LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
+    LRN_TRY(refine_inverses(L.a32, n, L.g, s));
     LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.g.err), 0u, 64, s));
     LRN_HIP(hipEventRecord(e1, s));
     int info = 0, err = 0;
@@ -2418,7 +2486,7 @@ LRN_API int lrn_getrs_f32f64_workspace_size(int64_t n, size_t* bytes)
 {
     if (n <= 0) return -1;
     if (bytes == nullptr) return -2;
-    *bytes = getrs_ws_bytes<double>(n, 1);
+    *bytes = f32f64_ws_bytes(n);
     return LRN_OK;
 }
 
@@ -2430,11 +2498,12 @@ LRN_API int lrn_getrs_f32f64(const float* lu, int64_t n, const int64_t* pivot, d
     if (pivot == nullptr) return -3;
     if (b == nullptr) return -4;
     if (n > kMaxOrder) return LRN_E_UNSUPPORTED;
-    if (workspace == nullptr || workspace_size < getrs_ws_bytes<double>(n, 1)) return -5;
+    if (workspace == nullptr || workspace_size < f32f64_ws_bytes(n)) return -5;
     hipStream_t s = static_cast<hipStream_t>(stream);
     Arena ar(workspace, workspace_size);
-    const auto L = carve_getrs<double>(ar, n, 1);
+    const auto L = carve_f32f64(ar, n);
     LRN_TRY(getrs_perm<double>(n, reinterpret_cast<const long long*>(pivot), L, s));
+    LRN_TRY(refine_inverses(lu, n, L, s));
     return getrs_subst<double, float>(lu, n, b, 1, 1, L, nullptr, s);
 }
 
@@ -2627,7 +2696,7 @@ LRN_API int lrn_debug_set_spin_limit(int64_t limit)
 
 LRN_API int lrn_set_refine_solver(int chain)
 {
-    if (chain < 0 || chain > 2) return -1;
+    if (chain < 0 || chain > 4) return -1;
     g_chain_solve.store(chain);
     return LRN_OK;
 }

@@ -8,6 +8,8 @@
 namespace lrn {
 
 constexpr int kWave = 64;
+// Accelerator complex dies of the MI355X: consecutive workgroup ids are dispatched to them in turn.
+constexpr int kXcds = 8;
 
 // Smallest normal number of the type: the reference divides instead of multiplying by the
 // reciprocal below it (src/matrix_operation/cpu/native/lu_decomposition_native.cpp:212-228).

@@ -29,7 +29,6 @@ constexpr int kG64BM = 128, kG64BN = 128, kG64BK = 16;
 // tri argument of the trailing-update GEMMs that disables the lower-triangle mode
 constexpr long long kNoTriangle = -(1ll << 62);
 typedef double g64v2 __attribute__((ext_vector_type(2)));  // native vector: stays in registers
-constexpr int kXcds = 8;
 constexpr int kGroupRows = 8;
 
 __device__ __forceinline__ double mfma4x4(double a, double b, double c)

@@ -351,6 +351,202 @@ __global__ __launch_bounds__(256) void trsv1_kernel(const TF* __restrict__ LU, l
     }
 }
 
+// Single right-hand side in the inverse form (the mixed solver's refinement, FP32 factors TF
+// applied in the working type T). Same 64-row blocks, thread layout and tagged-chunk hand-off
+// as trsv1_kernel, with the inverses of the diagonal blocks (dinvT, stored transposed as by
+// trsm_diag_inv_kernel: element b * 4096 + j * 64 + i is inv(D_b)[i][j]) and the products
+// M1_b = inv(D_b) A[b][p1], M2_b = inv(D_b) A[b][p2] with the two chain predecessors
+// p1 = b -+ 1, p2 = b -+ 2 (pred1T, pred2T, same layout, trsv_pred_kernel) computed once per
+// factorisation. Block b is
+//     x_b = z_b - M1_b x_p1 - M2_b x_p2,
+//     z_b = inv(D_b) (rhs_b - sum of A[b][j] x_j over the blocks before p2)
+//  * the off-diagonal blocks before the last three are taken kTrsvGroup at a time: the factor
+//    loads of a group and the polls of its solved values (one chunk per block per lane of
+//    wave 0) are all in flight together, one memory round trip per group;
+//  * the block before p2 is taken alone, so z_b is ready two hand-offs before it is needed;
+//  * x_p1 (and x_p2, arrived one hand-off earlier): each wave polls the values itself, so the
+//    hand-off is followed by 32 products, two lane exchanges and the store, with no workgroup
+//    barrier and no substitution.
+// Thread (r, part) holds row r, columns 16 part .. 16 part + 15, of inv(D_b), M1_b and M2_b in
+// registers from the start. Products of the blocks before p2 are accumulated in block order.
+// xcd_per > 0: workgroup i solves chain position (i % 8) * xcd_per + i / 8 (grid 8 * xcd_per,
+// positions >= nblk exit), so each run of xcd_per consecutive blocks is solved on one XCD
+// (workgroup i runs on XCD i % 8); every workgroup must then be resident at once (checked by
+// the host). xcd_per = 0: position i (the dispatch order, no residency needed).
+template <typename TF, int kPer>
+__device__ __forceinline__ void trsv1_load_raw(TF (&lreg)[kPer], const TF* __restrict__ lrow, long long c0, long long n,
+    bool row_ok)
+{
+    if (row_ok && c0 + kPer <= n) {
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) lreg[e] = lrow[c0 + e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) lreg[e] = (row_ok && c0 + e < n) ? lrow[c0 + e] : TF(0);
+    }
+}
+
+constexpr int kTrsvGroup = 8;  // off-diagonal blocks per group: 8 x 16 factor entries per thread
+
+template <typename TF, typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv1_inv_kernel(const TF* __restrict__ LU, long long lda, long long n,
+    const int* __restrict__ perm, const T* __restrict__ Bin, long long ldb_in, T* __restrict__ Out, long long ld_out,
+    Chunk* __restrict__ xch, unsigned tag, unsigned spin_limit, int* __restrict__ err,
+    const int* __restrict__ skip_if_nonzero, const T* __restrict__ dinvT, const T* __restrict__ pred1T,
+    const T* __restrict__ pred2T, int xcd_per)
+{
+    if (skip_if_nonzero != nullptr && *skip_if_nonzero != 0) return;
+    constexpr int TB = kSolveTB;
+    constexpr int kPer = TB / 4;
+    constexpr int G = kTrsvGroup;
+    __shared__ T Xs[TB];
+    __shared__ T Xg[2][G][TB];
+    __shared__ T Xv[TB];
+    __shared__ T Xw[4][2][TB];  // each wave's copy of the two predecessors' values
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const long long nblk = (n + TB - 1) / TB;
+    const long long chain_pos = xcd_per > 0
+        ? static_cast<long long>(blockIdx.x % kXcds) * xcd_per + blockIdx.x / kXcds
+        : static_cast<long long>(blockIdx.x);
+    if (chain_pos >= nblk) return;
+    const long long b = kLower ? chain_pos : (nblk - 1 - chain_pos);
+    const long long r0 = b * TB;
+    const int rows = static_cast<int>(min(static_cast<long long>(TB), n - r0));
+    const int r = tid >> 2, part = tid & 3;
+    const bool row_ok = r < rows;
+    const TF* lrow = LU + (r0 + r) * lda;
+
+    // number of earlier blocks in the chain order; the q-th of them is jblk(q)
+    const long long jcount = kLower ? b : (nblk - 1 - b);
+    auto jblk = [&](long long q) -> long long { return kLower ? q : (nblk - 1 - q); };
+
+    T dinv_reg[kPer], m1_reg[kPer], m2_reg[kPer];
+    {
+        const T* dp = dinvT + b * (TB * TB) + r;
+        const T* m1 = pred1T + b * (TB * TB) + r;
+        const T* m2 = pred2T + b * (TB * TB) + r;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            dinv_reg[e] = dp[(part * kPer + e) * TB];
+            m1_reg[e] = jcount > 0 ? m1[(part * kPer + e) * TB] : T(0);
+            m2_reg[e] = jcount > 1 ? m2[(part * kPer + e) * TB] : T(0);
+        }
+    }
+    T rhs = T(0);
+    if (part == 0 && row_ok) rhs = Bin[(kLower ? static_cast<long long>(perm[r0 + r]) : (r0 + r)) * ldb_in];
+
+    T part_sum = T(0);
+    const long long q_alone = jcount - 3;  // the block before the two predecessors (none when < 0)
+    TF lfin[kPer];
+    if (q_alone >= 0) trsv1_load_raw<TF, kPer>(lfin, lrow, jblk(q_alone) * TB + part * kPer, n, row_ok);
+    int gbuf = 0;
+    for (long long q0 = 0; q0 < q_alone; q0 += G, gbuf ^= 1) {
+        const int cnt = static_cast<int>(min(static_cast<long long>(G), q_alone - q0));
+        TF lg[G][kPer];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (g < cnt) {
+                trsv1_load_raw<TF, kPer>(lg[g], lrow, jblk(q0 + g) * TB + part * kPer, n, row_ok);
+            } else {
+#pragma unroll
+                for (int e = 0; e < kPer; ++e) lg[g][e] = TF(0);
+            }
+        }
+        if (tid < 64) {
+            // the chunks this lane needs; padding entries repeat a needed chunk (row 0 of the
+            // group's first block always exists)
+            const Chunk* pc[G];
+            const Chunk* const first = xch + jblk(q0) * TB;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const long long row = jblk(q0 + g) * TB + lane;
+                pc[g] = (g < cnt && row < n) ? xch + row : first;
+            }
+            u32x4 c[G];
+            spin_chunks<G>(c, pc, tag, spin_limit, err);
+#pragma unroll
+            for (int g = 0; g < G; ++g) Xg[gbuf][g][lane] = static_cast<T>(chunk_value(c[g]));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (g < cnt) {
+#pragma unroll
+                for (int e = 0; e < kPer; ++e)
+                    part_sum = fma(static_cast<T>(lg[g][e]), Xg[gbuf][g][part * kPer + e], part_sum);
+            }
+        }
+    }
+    if (q_alone >= 0) {
+        if (tid < 64) {
+            const long long row = jblk(q_alone) * TB + lane;
+            T v = T(0);
+            if (row < n) {
+                u32x4 c;
+                spin_chunk1(c, xch + row, tag, spin_limit, err);
+                v = static_cast<T>(chunk_value(c));
+            }
+            Xs[lane] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) part_sum = fma(static_cast<T>(lfin[e]), Xs[part * kPer + e], part_sum);
+    }
+    // the four partial sums of a row sit in adjacent lanes
+    part_sum += __shfl_xor(part_sum, 1, 64);
+    part_sum += __shfl_xor(part_sum, 2, 64);
+    if (part == 0) Xv[r] = row_ok ? rhs - part_sum : T(0);
+    __syncthreads();
+
+    // z_b (the inverse's padding rows and columns are zero)
+    T x = T(0);
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) x = fma(dinv_reg[e], Xv[part * kPer + e], x);
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    if (jcount > 0) {
+        // x_b = z_b - M1_b x_p1 - M2_b x_p2: every wave polls the 2 x 64 values itself (x_p2
+        // has normally arrived long before)
+        const long long prow1 = jblk(jcount - 1) * TB + lane;
+        const long long prow2 = jcount > 1 ? jblk(jcount - 2) * TB + lane : n;
+        const bool has2 = prow2 < n;
+        T v1 = T(0), v2 = T(0);
+        if (prow1 < n) {
+            const Chunk* pc[2] = {xch + prow1, xch + (has2 ? prow2 : prow1)};
+            u32x4 c[2];
+            spin_chunks<2>(c, pc, tag, spin_limit, err);
+            v1 = static_cast<T>(chunk_value(c[0]));
+            if (has2) v2 = static_cast<T>(chunk_value(c[1]));
+        } else if (has2) {
+            u32x4 c;
+            spin_chunk1(c, xch + prow2, tag, spin_limit, err);
+            v2 = static_cast<T>(chunk_value(c));
+        }
+        Xw[wid][0][lane] = v1;
+        Xw[wid][1][lane] = v2;
+        // the wave's own LDS writes, visible to its other lanes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        T d1 = T(0), d2 = T(0);
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            d1 = fma(m1_reg[e], Xw[wid][0][part * kPer + e], d1);
+            d2 = fma(m2_reg[e], Xw[wid][1][part * kPer + e], d2);
+        }
+        T d = d1 + d2;
+        d += __shfl_xor(d, 1, 64);
+        d += __shfl_xor(d, 2, 64);
+        x -= d;
+    }
+    if (part == 0 && row_ok) {
+        Out[(r0 + r) * ld_out] = x;
+        st_chunk(xch + r0 + r, static_cast<double>(x), static_cast<int>(r0 + r), tag);
+    }
+}
+
 // Single right-hand side with FP32 factors applied in the working type T (the refinement of
 // the mixed solver): 256-row workgroups, so the chain of hand-offs between workgroups has
 // n/256 links instead of n/64.

@@ -14,11 +14,11 @@ long long trsm_dinv_elems(long long n) { return (n + kTsRows - 1) / kTsRows * kT
 
 namespace {
 
-template <typename T, bool kLower, bool kUnit, bool kTrans>
-int inv_go(const T* a, long long lda, long long n, T* dinv, const int* skip, hipStream_t s)
+template <typename T, bool kLower, bool kUnit, bool kTrans, typename TA = T>
+int inv_go(const TA* a, long long lda, long long n, T* dinv, const int* skip, hipStream_t s)
 {
     const unsigned nblk = static_cast<unsigned>((n + kTsRows - 1) / kTsRows);
-    hipLaunchKernelGGL((trsm_diag_inv_kernel<T, kLower, kUnit, kTrans>), dim3(nblk, 4), dim3(256), 0, s, a, lda, n,
+    hipLaunchKernelGGL((trsm_diag_inv_kernel<TA, T, kLower, kUnit, kTrans>), dim3(nblk, 4), dim3(256), 0, s, a, lda, n,
         dinv, skip);
     return hipGetLastError() == hipSuccess ? LRN_OK : LRN_E_HIP;
 }
@@ -55,6 +55,29 @@ int trsm_inv_launch(bool lower, bool unit, bool trans, const T* a, long long lda
                          : inv_go<T, false, true, false>(a, lda, n, dinv, skip, s))
                 : (trans ? inv_go<T, false, false, true>(a, lda, n, dinv, skip, s)
                          : inv_go<T, false, false, false>(a, lda, n, dinv, skip, s));
+}
+
+int trsm_inv_widen_launch(bool lower, bool unit, const float* a, long long lda, long long n, double* dinv,
+    const int* skip, hipStream_t s)
+{
+    if (lower)
+        return unit ? inv_go<double, true, true, false, float>(a, lda, n, dinv, skip, s)
+                    : inv_go<double, true, false, false, float>(a, lda, n, dinv, skip, s);
+    return unit ? inv_go<double, false, true, false, float>(a, lda, n, dinv, skip, s)
+                : inv_go<double, false, false, false, float>(a, lda, n, dinv, skip, s);
+}
+
+int trsv_pred_widen_launch(bool lower, int dist, const float* a, long long lda, long long n, const double* dinv,
+    double* mt, const int* skip, hipStream_t s)
+{
+    const unsigned nblk = static_cast<unsigned>((n + kTsRows - 1) / kTsRows);
+    if (lower)
+        hipLaunchKernelGGL((trsv_pred_kernel<float, double, true>), dim3(nblk), dim3(256), 0, s, a, lda, n, dinv, mt, dist,
+            skip);
+    else
+        hipLaunchKernelGGL((trsv_pred_kernel<float, double, false>), dim3(nblk), dim3(256), 0, s, a, lda, n, dinv, mt, dist,
+            skip);
+    return hipGetLastError() == hipSuccess ? LRN_OK : LRN_E_HIP;
 }
 
 template <typename T>

@@ -88,9 +88,11 @@ __device__ __forceinline__ void ts_load_x_tile(T (*dst)[kTsRows + 2], const T* _
 // dinvT[b * 4096 + j * 64 + i] = inv(D_b)[i][j]. Padding rows and columns of the last block
 // are zero. grid (nblk, 4): workgroup (b, g) computes columns 16 g .. 16 g + 15 of inv(D_b),
 // wave w of it four of them, one column at a time with lane = row: forward (backward)
-// substitution on the unit vector, each solved entry broadcast with a lane read.
-template <typename T, bool kLower, bool kUnit, bool kTrans>
-__global__ __launch_bounds__(256) void trsm_diag_inv_kernel(const T* __restrict__ A, long long lda, long long n,
+// substitution on the unit vector, each solved entry broadcast with a lane read. The entries of
+// A (type TA) are widened to the arithmetic type T as they are read (FP32 factors inverted in
+// FP64 for the mixed solver's refinement).
+template <typename TA, typename T, bool kLower, bool kUnit, bool kTrans>
+__global__ __launch_bounds__(256) void trsm_diag_inv_kernel(const TA* __restrict__ A, long long lda, long long n,
     T* __restrict__ dinvT, const int* __restrict__ skip)
 {
     if (skip != nullptr && *skip != 0) return;
@@ -98,12 +100,23 @@ __global__ __launch_bounds__(256) void trsm_diag_inv_kernel(const T* __restrict_
     const long long b = blockIdx.x;
     const long long r0 = b * kTsRows;
     const int rows = static_cast<int>(min(static_cast<long long>(kTsRows), n - r0));
-    for (int idx = threadIdx.x; idx < kTsRows * kTsRows; idx += 256) {
-        // consecutive threads read consecutive addresses of A in either orientation
-        const int r = kTrans ? (idx & 63) : (idx >> 6);
-        const int c = kTrans ? (idx >> 6) : (idx & 63);
-        const bool in = r < rows && c < rows && (kLower ? (c < r || (!kUnit && c == r)) : (c > r || (!kUnit && c == r)));
-        Ds[r][c] = in ? op_coef<T, kTrans>(A, lda, r0 + r, r0 + c) : T(0);
+    {
+        // consecutive threads read consecutive addresses of A in either orientation; all 16
+        // loads of a thread in flight before any is written to LDS
+        T v[kTsPer];
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            const int r = kTrans ? (idx & 63) : (idx >> 6);
+            const int c = kTrans ? (idx >> 6) : (idx & 63);
+            const bool in = r < rows && c < rows && (kLower ? (c < r || (!kUnit && c == r)) : (c > r || (!kUnit && c == r)));
+            v[q] = in ? static_cast<T>(op_coef<TA, kTrans>(A, lda, r0 + r, r0 + c)) : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            Ds[kTrans ? (idx & 63) : (idx >> 6)][kTrans ? (idx >> 6) : (idx & 63)] = v[q];
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -134,6 +147,58 @@ __global__ __launch_bounds__(256) void trsm_diag_inv_kernel(const T* __restrict_
         }
         dinvT[b * kTsInvElems + j * kTsRows + lane] = (lane < rows && j < rows) ? x : T(0);
     }
+}
+
+// M_b = inv(D_b) A[b][p] for every 64-row block b with a chain predecessor p at distance dist
+// (p = b - dist when kLower, else b + dist), from the stored inverses (dinvT, as written by
+// trsm_diag_inv_kernel) and
+// the off-diagonal block of A (type TA, widened to T), stored in the same transposed layout:
+// MT[b * 4096 + j * 64 + i] = M_b[i][j]; padding rows and columns are zero; blocks without a
+// predecessor are not written. One workgroup per block, both operands in LDS, 16 results per
+// thread.
+template <typename TA, typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_pred_kernel(const TA* __restrict__ A, long long lda, long long n,
+    const T* __restrict__ dinvT, T* __restrict__ MT, int dist, const int* __restrict__ skip)
+{
+    if (skip != nullptr && *skip != 0) return;
+    __shared__ T Di[kTsRows][kTsRows + 1];  // Di[k][i] = inv(D_b)[i][k]
+    __shared__ T Ap[kTsRows][kTsRows + 1];  // Ap[k][j] = A[r0 + k][c0 + j]
+    const long long nblk = (n + kTsRows - 1) / kTsRows;
+    const long long b = blockIdx.x;
+    const long long p = kLower ? b - dist : b + dist;
+    if (p < 0 || p >= nblk) return;
+    const long long r0 = b * kTsRows, c0 = p * kTsRows;
+    const int rows = static_cast<int>(min(static_cast<long long>(kTsRows), n - r0));
+    const int cols = static_cast<int>(min(static_cast<long long>(kTsRows), n - c0));
+    const int tid = threadIdx.x;
+    {
+        T v[kTsPer], w[kTsPer];
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) {
+            const int idx = tid + 256 * q;
+            const int k = idx >> 6, j = idx & 63;
+            v[q] = dinvT[b * kTsInvElems + idx];
+            w[q] = (k < rows && j < cols) ? static_cast<T>(A[(r0 + k) * lda + c0 + j]) : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) {
+            const int idx = tid + 256 * q;
+            Di[idx >> 6][idx & 63] = v[q];
+            Ap[idx >> 6][idx & 63] = w[q];
+        }
+    }
+    __syncthreads();
+    const int i = tid & 63, j0 = tid >> 6;
+    T acc[kTsPer];
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) acc[q] = T(0);
+    for (int k = 0; k < kTsRows; ++k) {
+        const T d = Di[k][i];
+#pragma unroll
+        for (int q = 0; q < kTsPer; ++q) acc[q] = fma(d, Ap[k][j0 + 4 * q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kTsPer; ++q) MT[b * kTsInvElems + (j0 + 4 * q) * kTsRows + i] = acc[q];
 }
 
 // Y[i][c] = B[src(i)][c] for i < n, c < nc (src(i) = perm[i], or i when perm is null);

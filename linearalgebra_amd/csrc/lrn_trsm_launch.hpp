@@ -17,6 +17,16 @@ template <typename T>
 int trsm_inv_launch(bool lower, bool unit, bool trans, const T* a, long long lda, long long n, T* dinv,
     const int* skip, hipStream_t s);
 
+// The same inverses of FP32 triangular factors (not transposed), computed and stored in FP64.
+int trsm_inv_widen_launch(bool lower, bool unit, const float* a, long long lda, long long n, double* dinv,
+    const int* skip, hipStream_t s);
+
+// M_b = inv(D_b) A[b][b -+ dist] of every 64-row block b of FP32 triangular factors (a
+// predecessor in a forward / backward single-RHS substitution), from the inverses of
+// trsm_inv_widen_launch, in FP64, same layout and size (trsm_dinv_elems(n)).
+int trsv_pred_widen_launch(bool lower, int dist, const float* a, long long lda, long long n, const double* dinv,
+    double* mt, const int* skip, hipStream_t s);
+
 // X = op(A)^-1 X in place (X n x nrhs, row stride ldx) with the inverses from trsm_inv_launch
 // (same lower / trans flags): one launch per 64-row strip.
 template <typename T>

@@ -78,10 +78,16 @@ def getrf_tournament(m: torch.Tensor, pivot: torch.Tensor | None = None, stream=
     return pivot, int(info.cpu().item())
 
 
+DEFAULT_REFINE_SOLVER = 4
+
+
 def set_refine_solver(chain: int) -> None:
-    """Triangular solves of the refinement (process-wide): 0 / False = the 64-row kernel of the
-    plain single-RHS solve (the default), 1 / True = 256-row chained kernel, 2 = 128-row
-    chained kernel."""
+    """Triangular solves of the refinement (process-wide): 0 / False = the 64-row substitution
+    kernel of the plain single-RHS solve, 1 / True = 256-row chained kernel, 2 = 128-row
+    chained kernel, 3 = 64-row blocks in the inverse form (each diagonal block and the two
+    preceding off-diagonal blocks applied through products with the block's inverse, computed
+    in FP64 once per factorisation), 4 = mode 3 with runs of consecutive blocks kept on one XCD
+    (the default, ``DEFAULT_REFINE_SOLVER``)."""
     _check(_lib.load().lrn_set_refine_solver(int(chain)), "set_refine_solver")
 
 

@@ -82,6 +82,34 @@ def test_gesv_mixed_conditioned(n, kappa, dev):
     assert fe <= max(1e-12, kappa * n * np.finfo(np.float64).eps)
 
 
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+@pytest.mark.parametrize("solver", [3, 4])
+def test_gesv_mixed_refine_solvers(solver, kappa, dev):
+    """The refinement's substitution in the inverse form (modes 3, 4) reaches the FP64 backward
+    error by LU-IR like the 64-row substitution (mode 0), in about as many steps."""
+    import torch
+    import linearalgebra_amd as la
+    from linearalgebra_amd import mixed as lam
+    n = 3000
+    a = O.conditioned(n, kappa, seed=91)
+    b = O.gaussian(n, 1, seed=92).ravel()
+    ta, tb = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+    res = {}
+    try:
+        for mode in (0, solver):
+            lam.set_refine_solver(mode)
+            x, st = la.gesv_mixed(ta, tb, tol=1e-14, max_iter=30, restart=40)
+            res[mode] = (O.backward_error(a, x.cpu().numpy(), b), st)
+    finally:
+        lam.set_refine_solver(lam.DEFAULT_REFINE_SOLVER)
+    for mode, (berr, st) in res.items():
+        print(f"refine solver {mode} kappa={kappa:.0e} path={st['path_name']} ir={st['ir_iterations']} "
+              f"gmres={st['gmres_iterations']} berr={berr:.2e}", flush=True)
+        assert berr <= 2e-14
+        assert st["path"] == 0
+    assert res[solver][1]["ir_iterations"] <= res[0][1]["ir_iterations"] + 1
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_gesv_mixed_update_modes(mode, dev):
     """Every trailing-update mode (1, 2, 3 bf16 planes; FP32 MFMA) reaches the FP64 backward

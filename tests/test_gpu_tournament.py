@@ -126,14 +126,14 @@ def test_mixed_solver_tournament_vs_partial(kappa, dev):
     assert out[True][1]["ir_iterations"] <= out[False][1]["ir_iterations"] + 3
 
 
-@pytest.mark.parametrize("chain", [0, 1, 2])
+@pytest.mark.parametrize("chain", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 128, 129, 255, 256, 257, 700, 1000, 3000])
 def test_refine_substitution(n, chain, dev):
     """The refinement's triangular solves (FP32 factors, FP64 arithmetic) against the same
     solves done in FP64 on the host with the factors rounded up exactly."""
     import scipy.linalg as sl
     import torch
-    from linearalgebra_amd.mixed import getrs_f32f64, set_refine_solver
+    from linearalgebra_amd.mixed import DEFAULT_REFINE_SOLVER, getrs_f32f64, set_refine_solver
     rng = np.random.default_rng(n + 1000)
     a = rng.standard_normal((n, n)).astype(np.float32)
     lu, piv, info = _factor(a, dev)
@@ -145,7 +145,7 @@ def test_refine_substitution(n, chain, dev):
         getrs_f32f64(lu, piv, tb)
         torch.cuda.synchronize()
     finally:
-        set_refine_solver(0)
+        set_refine_solver(DEFAULT_REFINE_SOLVER)
     f = lu.cpu().numpy().astype(np.float64)
     perm = O.pivots_to_perm(piv.cpu().numpy())
     y = sl.solve_triangular(f, b[perm], lower=True, unit_diagonal=True)
