@@ -509,7 +509,7 @@ int launch_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, 
             const long long per_xcd = (tm * tn + kXcds - 1) / kXcds;
             int dev = 0;
             LRN_HIP(hipGetDevice(&dev));
-            const long long cap = 2ll * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
+            const long long cap = static_cast<long long>(kG32PerCu) * (cus > 0 ? cus : device_props(dev).cus) / kXcds * kXcds;
             const dim3 grid(static_cast<unsigned>(std::min(per_xcd * kXcds, cap)));
             TraceScope ts(kTrGemmF32, gemm_flops(m, nn, K, tri, kG32BM, kG32BN), s);
             if (check)

@@ -2,9 +2,13 @@
 // v_mfma_f32_16x16x4_f32 (the FP32 factorisation of the mixed solver).
 //
 // Tiling: 256 threads = 4 waves (2 x 2); a wave owns 64 x 64 of C as 4 x 4 tiles of 16 x 16
-// (64 FP32 accumulators per lane). K tiles of BK = 32 are staged k-major in LDS (A
+// (64 FP32 accumulators per lane). K tiles of BK = 16 are staged k-major in LDS (A
 // transposed on the way in, B as is), double-buffered with one barrier per K tile; the
-// float4 global loads of tile t+1 are issued before the 128 matrix instructions of tile t.
+// float4 global loads of tile t+1 are issued before the 64 matrix instructions of tile t.
+// BK = 16 keeps the LDS at 33 KB per workgroup, so three workgroups share a compute unit and
+// hide each other's barriers and C traffic (measured at C[16128 x 16128] -= A[16128 x 256]
+// B[256 x 16128]: 97 TF/s with three workgroups per unit against 91 TF/s for BK = 32 with two;
+// BK = 64 with one: 67 TF/s; the same products in the same order either way).
 // Persistent, XCD-aware grid (xcd_tile of lrn_gemm_f64.hpp): each XCD walks a contiguous run
 // of C tiles in groups of tile rows, so the A and B panels a group shares stay in its L2.
 // The product is accumulated from zero and subtracted from C once (delta-then-subtract, as
@@ -15,18 +19,21 @@
 
 namespace lrn {
 
-constexpr int kG32BM = 128, kG32BN = 128, kG32BK = 32;
+constexpr int kG32BM = 128, kG32BN = 128, kG32BK = 16;
+constexpr int kG32PerCu = 3;  // workgroups per compute unit the persistent grid is sized for
 typedef float g32v4 __attribute__((ext_vector_type(4)));  // native vector: stays in registers
 
-// Global -> register staging of one K tile: 4 float4 of A and 4 float4 of B per thread.
+// Global -> register staging of one K tile: 2 float4 of A and 2 float4 of B per thread.
+constexpr int kG32Ld = kG32BK / 8;
 template <bool kCheck>
-__device__ __forceinline__ void g32_load(g32v4 (&ra)[4], g32v4 (&rb)[4], const float* __restrict__ A, long long lda,
-    const float* __restrict__ B, long long ldb, long long row0, long long col0, int k0, int M, int N, int K, int tid)
+__device__ __forceinline__ void g32_load(g32v4 (&ra)[kG32Ld], g32v4 (&rb)[kG32Ld], const float* __restrict__ A,
+    long long lda, const float* __restrict__ B, long long ldb, long long row0, long long col0, int k0, int M, int N, int K,
+    int tid)
 {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = tid + q * 256;  // 128 rows x 8 float4
-        const int r = idx >> 3, c = (idx & 7) * 4;
+    for (int q = 0; q < kG32Ld; ++q) {
+        const int idx = tid + q * 256;  // 128 rows x 4 float4
+        const int r = idx >> 2, c = (idx & 3) * 4;
         const long long gr = row0 + r;
         const int gk = k0 + c;
         if (!kCheck) {
@@ -44,8 +51,8 @@ __device__ __forceinline__ void g32_load(g32v4 (&ra)[4], g32v4 (&rb)[4], const f
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = tid + q * 256;  // 32 rows x 32 float4
+    for (int q = 0; q < kG32Ld; ++q) {
+        const int idx = tid + q * 256;  // 16 rows x 32 float4
         const int r = idx >> 5, c = (idx & 31) * 4;
         const int gk = k0 + r;
         const long long gc = col0 + c;
@@ -65,10 +72,10 @@ __device__ __forceinline__ void g32_load(g32v4 (&ra)[4], g32v4 (&rb)[4], const f
     }
 }
 
-// kCheck = false requires M, N multiples of 128, K a multiple of 32 and 16-byte aligned rows
+// kCheck = false requires M, N multiples of 128, K a multiple of 16 and 16-byte aligned rows
 // (the host launches interior tiles this way and the ragged strips with kCheck = true).
 template <bool kCheck>
-__global__ __launch_bounds__(256) void gemm_sub_f32_kernel(const float* __restrict__ A, long long lda,
+__global__ __launch_bounds__(256, kG32PerCu) void gemm_sub_f32_kernel(const float* __restrict__ A, long long lda,
     const float* __restrict__ B, long long ldb, float* __restrict__ C, long long ldc, int M, int N, int K, int tm,
     int tn, int per_xcd, const int* __restrict__ skip, long long tri)
 {
@@ -101,22 +108,22 @@ __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(const float* __restri
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0.f, 0.f, 0.f, 0.f};
 
-        g32v4 ra[4], rb[4];
+        g32v4 ra[kG32Ld], rb[kG32Ld];
         const int nk = (K + BK - 1) / BK;
         g32_load<kCheck>(ra, rb, A, lda, B, ldb, row0, col0, 0, M, N, K, tid);
 
 #define LRN_G32_STORE(buf)                                                              \
     do {                                                                                \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q)                                   \
+        _Pragma("unroll") for (int q = 0; q < kG32Ld; ++q)                              \
         {                                                                               \
             const int idx = tid + q * 256;                                              \
-            const int r = idx >> 3, c = (idx & 7) * 4;                                  \
+            const int r = idx >> 2, c = (idx & 3) * 4;                                  \
             As[buf][c][r] = ra[q].x;                                                    \
             As[buf][c + 1][r] = ra[q].y;                                                \
             As[buf][c + 2][r] = ra[q].z;                                                \
             As[buf][c + 3][r] = ra[q].w;                                                \
         }                                                                               \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q)                                   \
+        _Pragma("unroll") for (int q = 0; q < kG32Ld; ++q)                              \
         {                                                                               \
             const int idx = tid + q * 256;                                              \
             const int r = idx >> 5, c = (idx & 31) * 4;                                 \
