@@ -409,22 +409,27 @@ __global__ __launch_bounds__(kBatThreads, 2) void batched_gesv_kernel(T* __restr
         __syncthreads();
         // -------------------------------------------------------------------- row order
         // write the rows in logical order, 32 columns at a time (all reads of a block before
-        // any of its writes)
-        for (int c0 = 0; c0 < n; c0 += NB) {
-            T rv[NB];
-            if (tid < n) {
-                const T* src = A + static_cast<long long>(S.phys[tid]) * n + c0;
+        // any of its writes); lane groups of 32 cover the 32 columns of one row, so each wave
+        // instruction moves two 32-element row pieces
+        {
+            constexpr int kRowsPer = kBatMaxN / (kBatThreads / NB);  // rows per thread per block
+            const int cc = tid & (NB - 1), rg = tid / NB;
+            for (int c0 = 0; c0 < n; c0 += NB) {
+                T rv[kRowsPer];
+                const bool col_ok = c0 + cc < n;
 #pragma unroll
-                for (int c = 0; c < NB; ++c) rv[c] = (c0 + c < n) ? src[c] : T(0);
-            }
-            __syncthreads();
-            if (tid < n) {
-                T* dst = A + static_cast<long long>(tid) * n + c0;
+                for (int q = 0; q < kRowsPer; ++q) {
+                    const int i = rg + (kBatThreads / NB) * q;
+                    rv[q] = (i < n && col_ok) ? A[static_cast<long long>(S.phys[i]) * n + c0 + cc] : T(0);
+                }
+                __syncthreads();
 #pragma unroll
-                for (int c = 0; c < NB; ++c)
-                    if (c0 + c < n) dst[c] = rv[c];
+                for (int q = 0; q < kRowsPer; ++q) {
+                    const int i = rg + (kBatThreads / NB) * q;
+                    if (i < n && col_ok) A[static_cast<long long>(i) * n + c0 + cc] = rv[q];
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
         const int inf = S.info;
 
