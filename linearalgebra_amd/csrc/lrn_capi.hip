@@ -1858,8 +1858,13 @@ struct MixedSolver {
     // out = beta * bb + alpha * A v
     int gemv(const double* v, const double* bb, double* out, double alpha, double beta)
     {
-        hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, v, bb, out,
-            alpha, beta, 0);
+        const bool vec = n % 2 == 0 && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(v)) % 16 == 0);
+        if (vec)
+            hipLaunchKernelGGL(gemv_rows_vec_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, v, bb,
+                out, alpha, beta);
+        else
+            hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, v, bb,
+                out, alpha, beta, 0);
         LRN_HIP(hipGetLastError());
         return LRN_OK;
     }
@@ -1871,13 +1876,18 @@ struct MixedSolver {
         // L.g.err is sticky over the whole refinement and read once at the end
         return getrs_subst<double, float>(L.a32, n, out, 1, 1, L.g, nullptr, s, true);
     }
-    // r = b - A x and the backward error
+    // r = b - A x and the backward error (both norms fetched with one synchronisation)
     int residual(double* berr)
     {
         LRN_TRY(gemv(x, b, L.r, -1.0, 1.0));
-        double rn = 0.0, xn = 0.0;
-        LRN_TRY(reduce(L.r, nullptr, 0, &rn));
-        LRN_TRY(reduce(x, nullptr, 0, &xn));
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, L.r, nullptr, n, 0, L.scal);
+        LRN_HIP(hipGetLastError());
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, x, nullptr, n, 0, L.scal + 1);
+        LRN_HIP(hipGetLastError());
+        double norms[2] = {0.0, 0.0};
+        LRN_HIP(hipMemcpyAsync(norms, L.scal, sizeof(norms), hipMemcpyDeviceToHost, s));
+        LRN_HIP(hipStreamSynchronize(s));
+        const double rn = norms[0], xn = norms[1];
         const double den = anorm * xn + bnorm;
         *berr = den > 0.0 ? rn / den : rn;
         return LRN_OK;
