@@ -12,9 +12,11 @@
 //   result   : D[lane >> 4][lane & 15]
 //
 // Tiling: 512 threads = 8 waves (4 x 2); a wave owns 32 x 64 of C as 8 x 4 such 4 x 16
-// tiles (32 FP64 accumulators per lane), small enough for four waves per SIMD. A and B tiles of BK = 16 are staged k-major in
-// LDS, double-buffered with one barrier per K tile; global loads of tile t+1 are issued
-// before the MFMAs of tile t. Workgroup-to-tile mapping is XCD-aware: consecutive workgroup
+// tiles (32 FP64 accumulators per lane), small enough for four waves per SIMD. A and B tiles of
+// BK = 8 are staged k-major in LDS, double-buffered with one barrier per K tile; global loads of
+// tile t+1 are issued before the MFMAs of tile t. (BK = 16 needs more registers than four waves
+// per SIMD leave and spilled 44 bytes per lane: 52.5 against 54.4 TF/s for BK = 8 at
+// C[16128 x 16128] -= A[16128 x 256] B[256 x 16128], with the same products in the same order.) Workgroup-to-tile mapping is XCD-aware: consecutive workgroup
 // ids go round-robin to the 8 XCDs, so each XCD receives a contiguous run of tiles, walked
 // in groups of 8 tile rows so the A and B panels a group shares stay in that XCD's L2.
 // The product is accumulated from zero and subtracted from C once (the delta-then-subtract
@@ -25,7 +27,8 @@
 
 namespace lrn {
 
-constexpr int kG64BM = 128, kG64BN = 128, kG64BK = 16;
+constexpr int kG64BM = 128, kG64BN = 128, kG64BK = 8;
+constexpr int kG64Ld = kG64BK / 8;  // 16-byte pairs of A and of B each thread stages per K tile
 // tri argument of the trailing-update GEMMs that disables the lower-triangle mode
 constexpr long long kNoTriangle = -(1ll << 62);
 typedef double g64v2 __attribute__((ext_vector_type(2)));  // native vector: stays in registers
@@ -51,16 +54,17 @@ __device__ __forceinline__ bool xcd_tile(int wg, int tm, int tn, int per_xcd, in
     return true;
 }
 
-// Global -> register staging of one K tile (BK = 16): 2 pairs of A and 2 pairs of B per thread.
+// Global -> register staging of one K tile (BK = 8): one pair of A and one pair of B per thread.
 // kCheck selects the bounds-checked form for edge tiles; interior tiles use plain 16-byte loads.
 template <bool kCheck, bool kVec>
-__device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const double* __restrict__ A, long long lda,
-    const double* __restrict__ B, long long ldb, long long row0, long long col0, int k0, int M, int N, int K, int tid)
+__device__ __forceinline__ void g64_load(g64v2 (&ra)[kG64Ld], g64v2 (&rb)[kG64Ld], const double* __restrict__ A,
+    long long lda, const double* __restrict__ B, long long ldb, long long row0, long long col0, int k0, int M, int N, int K,
+    int tid)
 {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int idx = tid + q * 512;  // 128 rows x 8 pairs
-        const int r = idx >> 3, c = (idx & 7) * 2;
+    for (int q = 0; q < kG64Ld; ++q) {
+        const int idx = tid + q * 512;  // 128 rows x BK/2 pairs
+        const int r = idx / (kG64BK / 2), c = (idx % (kG64BK / 2)) * 2;
         const long long gr = row0 + r;
         const int gk = k0 + c;
         if (!kCheck) {
@@ -80,8 +84,8 @@ __device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const d
         }
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int idx = tid + q * 512;  // 16 rows x 64 pairs
+    for (int q = 0; q < kG64Ld; ++q) {
+        const int idx = tid + q * 512;  // BK rows x 64 pairs
         const int r = idx >> 6, c = (idx & 63) * 2;
         const int gk = k0 + r;
         const long long gc = col0 + c;
@@ -103,7 +107,7 @@ __device__ __forceinline__ void g64_load(g64v2 (&ra)[2], g64v2 (&rb)[2], const d
     }
 }
 
-// 512 threads. kCheck = false requires M, N multiples of 128, K a multiple of 16 and 16-byte
+// 512 threads. kCheck = false requires M, N multiples of 128, K a multiple of 8 and 16-byte
 // aligned rows (the host launches interior tiles this way and edge strips with kCheck = true).
 template <bool kCheck>
 __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __restrict__ A, long long lda,
@@ -141,21 +145,21 @@ __global__ __launch_bounds__(512, 4) void gemm_sub_f64_kernel(const double* __re
 #pragma unroll
         for (int b = 0; b < CG; ++b) acc[a][b] = 0.0;
 
-    g64v2 ra[2], rb[2];
+    g64v2 ra[kG64Ld], rb[kG64Ld];
     const int nk = (K + BK - 1) / BK;
     g64_load<kCheck, false>(ra, rb, A, lda, B, ldb, row0, col0, 0, M, N, K, tid);
 
     // LDS store of the staged tile: A transposed to k-major, B as is
 #define LRN_G64_STORE(buf)                                                              \
     do {                                                                                \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                                   \
+        _Pragma("unroll") for (int q = 0; q < kG64Ld; ++q)                              \
         {                                                                               \
             const int idx = tid + q * 512;                                              \
-            const int r = idx >> 3, c = (idx & 7) * 2;                                  \
+            const int r = idx / (kG64BK / 2), c = (idx % (kG64BK / 2)) * 2;             \
             As[buf][c][r] = ra[q].x;                                                    \
             As[buf][c + 1][r] = ra[q].y;                                                \
         }                                                                               \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                                   \
+        _Pragma("unroll") for (int q = 0; q < kG64Ld; ++q)                              \
         {                                                                               \
             const int idx = tid + q * 512;                                              \
             const int r = idx >> 6, c = (idx & 63) * 2;                                 \

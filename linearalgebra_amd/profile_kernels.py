@@ -35,6 +35,8 @@ def main() -> None:
     p.add_argument("--count", type=int, default=10000)
     p.add_argument("--reps", type=int, default=5)
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--time", action="store_true",
+                   help="gemm_*: time each launch with HIP events on its stream and print the fastest")
     a = p.parse_args()
 
     import torch
@@ -57,10 +59,23 @@ def main() -> None:
         C = torch.randn(a.m, a.n, dtype=dtype, device=dev, generator=g)
         torch.cuda.synchronize()
         print(f"{a.kind}: {a.m} x {a.n} x {a.k}, {a.reps} launches", flush=True)
+        evs = []
         for _ in range(a.reps):
+            e0 = torch.cuda.Event(enable_timing=True) if a.time else None
+            e1 = torch.cuda.Event(enable_timing=True) if a.time else None
+            if a.time:
+                e0.record()
             rc = lib.lrn_debug_gemm_sub(code, vp(A), a.k, vp(B), a.n, vp(C), a.n, a.m, a.n, a.k, s)
+            if a.time:
+                e1.record()
+                evs.append((e0, e1))
             if rc != 0:
                 sys.exit(f"lrn_debug_gemm_sub returned {rc}")
+        if a.time:
+            torch.cuda.synchronize()
+            ms = min(x.elapsed_time(y) for x, y in evs)
+            print(f"{a.kind}: fastest launch {ms:.3f} ms = {2.0 * a.m * a.n * a.k / ms / 1e9:.1f} TF/s "
+                  f"(HIP events on the launch stream)", flush=True)
     elif a.kind == "trsm_f64":
         from .triangular import triangular_solver
         A = torch.randn(a.m, a.m, dtype=torch.float64, device=dev, generator=g) / a.m ** 0.5
