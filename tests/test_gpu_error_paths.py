@@ -5,7 +5,8 @@ LRN_E_TIMEOUT; the asynchronous solvers leave NaN in x instead of a wrong soluti
 inversion does not copy its (invalid) result over m. The limit is restored after each test.
 The many-right-hand-side solves run in each of their three forms: the chained substitution
 and the blocked form's chained kernel wait between workgroups (NaN expected); the blocked form
-with one launch per strip has no such wait and must solve correctly under the same limit."""
+with one launch per strip has no such wait and must solve correctly under the same limit. The
+mixed refinement's single-RHS substitutions (each form) leave NaN as well."""
 import numpy as np
 import pytest
 
@@ -162,3 +163,27 @@ def test_inversion_and_mixed_report_timeout(expired_waits, dev):
     bm = torch.from_numpy(O.gaussian(512, 1, seed=9)[:, 0]).to(dev)
     with pytest.raises(la.LinalgError):
         la.gesv_mixed(am, bm)
+
+
+@pytest.mark.parametrize("solver", [0, 3, 4])
+def test_refinement_substitution_leaves_nan(expired_waits, solver, dev):
+    """The mixed refinement's substitution kernels (the 64-row substitution and the inverse form
+    in dispatch and XCD-local order) give up their waits at once under the forced expiry,
+    raise the error word, finish, and the solve's result is poisoned with NaN."""
+    import torch
+    from linearalgebra_amd.mixed import DEFAULT_REFINE_SOLVER, getrs_f32f64, set_refine_solver
+    n = 3000
+    a = O.gaussian(n, seed=31).astype(np.float32)
+    # any FP32 factors serve: a diagonally dominant matrix read as its own L and U, P = I
+    a += np.float32(4 * n) * np.eye(n, dtype=np.float32)
+    lu = torch.from_numpy(a).to(dev)
+    piv = torch.arange(1, n + 1, dtype=torch.int64, device=dev)
+    b = torch.from_numpy(O.gaussian(n, 1, seed=32)[:, 0]).to(dev)
+    try:
+        set_refine_solver(solver)
+        getrs_f32f64(lu, piv, b)
+        torch.cuda.synchronize()
+    finally:
+        set_refine_solver(DEFAULT_REFINE_SOLVER)
+    assert np.isnan(b.cpu().numpy()).all()
+
