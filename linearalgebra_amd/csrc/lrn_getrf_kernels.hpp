@@ -276,9 +276,12 @@ constexpr int kSwapCols = 64;
 // (a row-move kernel followed by trsm_llnu_kernel would be two latency-bound launches on
 // little data).
 // Workgroups [0, na) take 64-column strips of [K, k) (moves only), workgroups [na, ..) strips
-// of [k+w, KB): moves, a barrier (the workgroup's own stores are then visible to it), then
-// one thread per column solves the strip's U12 rows in LDS in increasing pivot order with
-// every product rounded before its subtraction -- bitwise the result of trsm_llnu_kernel.
+// of [k+w, KB). All loads of a workgroup are issued together: the moved rows, and for the
+// right strips L11 and the strip's rows k..k+w-1 as they were; A12 after the moves is then
+// assembled in LDS (a moved row's values from the gathered rows, the others as loaded), so
+// the kernel makes one memory round trip before its stores. One thread per column then solves
+// its column in registers in increasing pivot order with every product rounded before its
+// subtraction -- bitwise the result of trsm_llnu_kernel.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long long lda,
     const int* __restrict__ rowmap, int W2, long long K, long long k, int w, long long KB, int na)
@@ -295,35 +298,18 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long
     const int cnt = rowmap[0];
     const int* dst = rowmap + 1;
     const int* src = rowmap + 1 + W2;
-    // every load of a phase is issued before the first of them is used: fully unrolled loops
-    // over the thread's share, through registers (a rolled loop waits for each load in turn)
+    // every load is issued before the first of them is used: fully unrolled loops over the
+    // thread's share, through registers (a rolled loop waits for each load in turn)
     constexpr int kMoves = 2 * W * kSwapCols / 256;
-    {
-        T v[kMoves];
-#pragma unroll
-        for (int t = 0; t < kMoves; ++t) {
-            const int idx = tid + 256 * t;
-            const int q = idx / kSwapCols, c = idx - q * kSwapCols;
-            v[t] = (q < cnt && c < ncols) ? A[static_cast<long long>(src[q]) * lda + cb + c] : T(0);
-        }
-#pragma unroll
-        for (int t = 0; t < kMoves; ++t) {
-            const int idx = tid + 256 * t;
-            buf[idx / kSwapCols][idx % kSwapCols] = v[t];
-        }
-    }
-    __syncthreads();
+    constexpr int kL = (W * W + 255) / 256, kX = (W * kSwapCols + 255) / 256;
+    T v[kMoves], lv[kL], xv[kX];
 #pragma unroll
     for (int t = 0; t < kMoves; ++t) {
         const int idx = tid + 256 * t;
         const int q = idx / kSwapCols, c = idx - q * kSwapCols;
-        if (q < cnt && c < ncols) A[static_cast<long long>(dst[q]) * lda + cb + c] = buf[q][c];
+        v[t] = (q < cnt && c < ncols) ? A[static_cast<long long>(src[q]) * lda + cb + c] : T(0);
     }
-    if (!right) return;  // uniform over the workgroup
-    __syncthreads();
-    {
-        constexpr int kL = (W * W + 255) / 256, kX = (W * kSwapCols + 255) / 256;
-        T lv[kL], xv[kX];
+    if (right) {
 #pragma unroll
         for (int t = 0; t < kL; ++t) {
             const int idx = tid + 256 * t;
@@ -336,6 +322,13 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long
             const int r = idx / kSwapCols, c = idx % kSwapCols;
             xv[t] = (idx < W * kSwapCols && r < w && c < ncols) ? A[(k + r) * lda + cb + c] : T(0);
         }
+    }
+#pragma unroll
+    for (int t = 0; t < kMoves; ++t) {
+        const int idx = tid + 256 * t;
+        buf[idx / kSwapCols][idx % kSwapCols] = v[t];
+    }
+    if (right) {
 #pragma unroll
         for (int t = 0; t < kL; ++t) {
             const int idx = tid + 256 * t;
@@ -348,14 +341,44 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long
         }
     }
     __syncthreads();
-    if (tid < ncols) {
-        const int c = tid;
-        for (int i = 1; i < w; ++i) {
-            T s = X[i][c];
-            for (int p = 0; p < i; ++p) s = s - L11[i][p] * X[p][c];
-            X[i][c] = s;
+#pragma unroll
+    for (int t = 0; t < kMoves; ++t) {
+        const int idx = tid + 256 * t;
+        const int q = idx / kSwapCols, c = idx - q * kSwapCols;
+        if (q < cnt && c < ncols) A[static_cast<long long>(dst[q]) * lda + cb + c] = buf[q][c];
+    }
+    if (!right) return;  // uniform over the workgroup
+    // rows k..k+w-1 that receive a moved row take it from the gathered rows (each row is the
+    // destination of at most one move)
+#pragma unroll
+    for (int t = 0; t < kMoves; ++t) {
+        const int idx = tid + 256 * t;
+        const int q = idx / kSwapCols, c = idx - q * kSwapCols;
+        if (q < cnt && c < ncols) {
+            const long long d = static_cast<long long>(dst[q]) - k;
+            if (d >= 0 && d < w) X[d][c] = buf[q][c];
         }
-        for (int i = 1; i < w; ++i) A[(k + i) * lda + cb + c] = X[i][c];
+    }
+    __syncthreads();
+    if (tid < ncols) {
+        // the column in registers (every index a compile-time constant); the L11 entries are
+        // broadcast reads off the chain of products and subtractions
+        const int c = tid;
+        T u[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) u[i] = (i < w) ? X[i][c] : T(0);
+#pragma unroll
+        for (int i = 1; i < W; ++i) {
+            if (i < w) {
+                T sum = u[i];
+#pragma unroll
+                for (int p = 0; p < i; ++p) sum = sum - L11[i][p] * u[p];
+                u[i] = sum;
+            }
+        }
+#pragma unroll
+        for (int i = 1; i < W; ++i)
+            if (i < w) A[(k + i) * lda + cb + c] = u[i];
     }
 }
 
@@ -778,6 +801,19 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, 
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
         for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = acc_t{0, 0, 0, 0};
+    // C's old values are loaded with the first operand tiles, so the tile costs one memory
+    // round trip instead of two (the block-column update has K = 32: a single operand stage)
+    T cpre[MI][NI][4];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long gr = row0 + wm * WM + mi * 16 + MF::row(lane, v);
+                const long long gc = col0 + wn * WN + ni * 16 + (lane & 15);
+                cpre[mi][ni][v] = (gr < M && gc < N) ? C[gr * ldc + gc] : T(0);
+            }
 
     constexpr int A_PER = BM * BK / 256;
     constexpr int B_PER = BK * BN / 256;
@@ -845,10 +881,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, 
             for (int v = 0; v < 4; ++v) {
                 const long long gr = row0 + wm * WM + mi * 16 + MF::row(lane, v);
                 const long long gc = col0 + wn * WN + ni * 16 + (lane & 15);
-                if (gr < M && gc < N) {
-                    T* cp = C + gr * ldc + gc;
-                    *cp = *cp - acc[mi][ni][v];
-                }
+                if (gr < M && gc < N) C[gr * ldc + gc] = cpre[mi][ni][v] - acc[mi][ni][v];
             }
 }
 
