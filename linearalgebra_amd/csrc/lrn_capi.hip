@@ -766,9 +766,14 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
                 // matrix cores, 64 x 64 tiles, the whole K = w in one LDS stage
                 const long long tm = (rows + 63) / 64, tn = (cols + 63) / 64;
                 TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
-                hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s,
-                    A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
-                    static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
+                if (w <= 32)
+                    hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32, true>), dim3(static_cast<unsigned>(tm * tn)), dim3(256),
+                        0, s, A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
+                        static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
+                else
+                    hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0,
+                        s, A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
+                        static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
                 LRN_HIP(hipGetLastError());
                 continue;
             }

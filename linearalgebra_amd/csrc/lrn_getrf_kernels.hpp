@@ -774,7 +774,9 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
 // (A[i][k] at lane i + 16k, B[k][j] at lane j + 16k) is one conflict-free LDS read.
 // The product is accumulated from zero and subtracted from C once at the end, the
 // reference's delta-then-subtract structure.
-template <typename T, int BM, int BN, int BK>
+// kOneStage: K <= BK (the block-column update, K = the inner panel width): no K loop, so the
+// staging registers are free before the products (fewer registers, more workgroups per unit).
+template <typename T, int BM, int BN, int BK, bool kOneStage = false>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, long long lda,
     const T* __restrict__ B, long long ldb, T* __restrict__ C, long long ldc, int M, int N, int K,
     int tiles_n)
@@ -852,12 +854,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, 
         }
     };
 
-    load_tiles(0);
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        __syncthreads();
-        store_tiles();
-        __syncthreads();
-        if (k0 + BK < K) load_tiles(k0 + BK);
+    auto products = [&]() {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T a[MI], bb[NI];
@@ -870,6 +867,20 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, 
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = MF::mma(a[mi], bb[ni], acc[mi][ni]);
+        }
+    };
+    load_tiles(0);
+    if constexpr (kOneStage) {
+        store_tiles();
+        __syncthreads();
+        products();
+    } else {
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            __syncthreads();
+            store_tiles();
+            __syncthreads();
+            if (k0 + BK < K) load_tiles(k0 + BK);
+            products();
         }
     }
 
