@@ -775,9 +775,11 @@ __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long 
 // The product is accumulated from zero and subtracted from C once at the end, the
 // reference's delta-then-subtract structure.
 // kOneStage: K <= BK (the block-column update, K = the inner panel width): no K loop, so the
-// staging registers are free before the products (fewer registers, more workgroups per unit).
+// staging registers are free before the products, and the register budget is set for five
+// (FP32) or four (FP64) workgroups per compute unit (no spills): the update runs on the 32
+// compute units of the panel stream, where resident workgroups, not arithmetic, set its time.
 template <typename T, int BM, int BN, int BK, bool kOneStage = false>
-__global__ __launch_bounds__(256) void gemm_sub_kernel(const T* __restrict__ A, long long lda,
+__global__ __launch_bounds__(256, kOneStage ? (sizeof(T) == 4 ? 5 : 4) : 1) void gemm_sub_kernel(const T* __restrict__ A, long long lda,
     const T* __restrict__ B, long long ldb, T* __restrict__ C, long long ldc, int M, int N, int K,
     int tiles_n)
 {
