@@ -805,7 +805,7 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
         return LRN_OK;
     }
     // one launch for the whole block column
-    // narrow column ranges (the next block column): 16 columns per workgroup for parallelism
+    // narrow column ranges (the next block column): 8 columns per workgroup for parallelism
     const bool narrow = c1 - c0 <= 256;
     const int tn = narrow ? kU12TNNarrow : kU12TN;
     const long long blocks = (c1 - c0 + tn - 1) / tn;

@@ -668,7 +668,7 @@ __global__ __launch_bounds__(256) void laswp_block_kernel(T* __restrict__ A, lon
 // result does not depend on the grouping, nor on TN (columns per workgroup: 64 for wide
 // column ranges; 16 for the next block column alone, which is on the look-ahead's critical
 // path and would otherwise give only a few workgroups).
-constexpr int kU12TN = 64, kU12TNNarrow = 16;
+constexpr int kU12TN = 64, kU12TNNarrow = 8;
 template <typename T, int TN>
 __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long long lda, long long K, int B,
     long long c0, long long c1)
