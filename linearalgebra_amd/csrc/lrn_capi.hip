@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kMaxW = 64;
 constexpr int kMaxInner = 16;  // inner panels per outer block column (row-map slots)
+constexpr int kRowmapSets = 3;  // block columns whose row maps are live at once
 // Polls allowed to every bounded inter-workgroup wait (lrn_debug_set_spin_limit; 0 makes every
 // wait expire at once, which tests use to force the error paths).
 std::atomic<unsigned> g_spin_limit{1u << 20};
@@ -110,9 +111,9 @@ template <typename T> GetrfLayout<T> carve_getrf(Arena& ar)
     L.ws.sidx = ar.next<long long>(2 * kPanelMaxWG);
     L.ws.srow = ar.next<T>(static_cast<size_t>(2) * kPanelMaxWG * kMaxW);
     L.ws.sdiag = ar.next<T>(2 * kMaxW);
-    // two sets of row maps: with the look-ahead the next block column is factored while the
-    // current one's maps are still being applied
-    L.ws.rowmap = ar.next<int>(static_cast<size_t>(2) * kMaxInner * (1 + 4 * kMaxW));
+    // three sets of row maps: with the look-ahead block column K+2 is factored while block
+    // column K's maps may still be applied to the far columns
+    L.ws.rowmap = ar.next<int>(static_cast<size_t>(kRowmapSets) * kMaxInner * (1 + 4 * kMaxW));
     L.ws.chunks = ar.next<Chunk>(static_cast<size_t>(2) * kPanel2MaxWG * 32);
     L.ws.dchunks = ar.next<Chunk>(2 * 32);
     return L;
@@ -419,17 +420,19 @@ int launch_trsm(T* A, long long lda, long long k, int w, long long c0, long long
     return LRN_OK;
 }
 
+// Inner panel [k, k+w)'s interchanges on the columns [K, k) (moves only) and, with its U12
+// rows, on the columns [rs, re).
 template <typename T, int W>
-int launch_laswp_trsm(T* A, long long lda, const int* rowmap, int W2, long long K, long long k, int w, long long KB,
-    hipStream_t s)
+int launch_laswp_trsm(T* A, long long lda, const int* rowmap, int W2, long long K, long long k, int w, long long rs,
+    long long re, hipStream_t s)
 {
     const long long na = (k - K + kSwapCols - 1) / kSwapCols;
-    const long long nb = (KB - (k + w) + kSwapCols - 1) / kSwapCols;
+    const long long nb = re > rs ? (re - rs + kSwapCols - 1) / kSwapCols : 0;
     if (na + nb <= 0) return LRN_OK;
-    const double cols = static_cast<double>((k - K) + (KB - (k + w)));
-    TraceScope ts(kTrLaswp, (2.0 * W2 * cols + static_cast<double>(w) * w + 2.0 * w * (KB - k - w)) * sizeof(T), s);
+    const double right = static_cast<double>(std::max(0ll, re - rs));
+    TraceScope ts(kTrLaswp, (2.0 * W2 * ((k - K) + right) + static_cast<double>(w) * w + 2.0 * w * right) * sizeof(T), s);
     hipLaunchKernelGGL((laswp_trsm_kernel<T, W>), dim3(static_cast<unsigned>(na + nb)), dim3(256), 0, s, A, lda, rowmap,
-        W2, K, k, w, KB, static_cast<int>(na));
+        W2, K, k, w, rs, re, static_cast<int>(na));
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }
@@ -654,6 +657,8 @@ struct AsideStream {
     hipEvent_t ready = nullptr;      // recorded on the narrow stream: next block column updated
     hipEvent_t done = nullptr;       // recorded on the panel stream: next block column factored
     hipEvent_t rest_done = nullptr;  // recorded on the GEMM stream: rest of the trailing update done
+    hipEvent_t step = nullptr;       // recorded on the panel stream after each inner panel
+    hipEvent_t ext_done = nullptr;   // recorded on the narrow stream: next block column's inner updates done
     std::mutex mu;
 };
 
@@ -679,6 +684,8 @@ void release_aside_streams()
             (void)hipEventDestroy(a->ready);
             (void)hipEventDestroy(a->done);
             (void)hipEventDestroy(a->rest_done);
+            (void)hipEventDestroy(a->step);
+            (void)hipEventDestroy(a->ext_done);
             (void)hipStreamDestroy(a->narrow);
             (void)hipStreamDestroy(a->panel);
             (void)hipStreamDestroy(a->gemm);
@@ -713,6 +720,8 @@ int aside_stream(int np, AsideStream** out)
     LRN_HIP(hipEventCreateWithFlags(&a->ready, hipEventDisableTiming));
     LRN_HIP(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
     LRN_HIP(hipEventCreateWithFlags(&a->rest_done, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->step, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->ext_done, hipEventDisableTiming));
     *out = a.get();
     per_dev[dev].push_back(std::move(a));
     static const bool at_exit_registered = (std::atexit(release_aside_streams) == 0);
@@ -726,15 +735,67 @@ std::atomic<int> g_lookahead{0};
 // 64 x 64 matrix-core tiles.
 std::atomic<int> g_block_update{0};
 
-// Factors block column [K, K+B) (rows K..n-1) inner panel by inner panel; each inner panel's
-// interchanges are applied inside the block column at once, and its L updates the rest of
-// the block column (TRSM + GEMM restricted to columns [K, K+B)).
+// A[k+w.., c0..c1) -= L21 U12 after inner panel [k, k+w): the update kind selected by
+// g_block_update. Every kind computes each entry independently of the column range, so an
+// update split over several ranges is bitwise one update of their union.
+template <typename T, int W>
+int launch_block_update(T* A, long long lda, long long n, long long k, int w, long long c0, long long c1, hipStream_t s)
+{
+    const long long rows = n - (k + w), cols = c1 - c0;
+    if (rows <= 0 || cols <= 0) return LRN_OK;
+    const int kind = g_block_update.load();
+    if (kind == 1)
+        return launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda, rows,
+            cols, w, s);
+    TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
+    if (kind == 2) {
+        // matrix cores, 64 x 64 tiles, the whole K = w in one LDS stage
+        const long long tm = (rows + 63) / 64, tn = (cols + 63) / 64;
+        if (w <= 32)
+            hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32, true>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0,
+                s, A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda,
+                static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
+        else
+            hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0, s,
+                A + (k + w) * lda + k, lda, A + k * lda + c0, lda, A + (k + w) * lda + c0, lda, static_cast<int>(rows),
+                static_cast<int>(cols), w, static_cast<int>(tn));
+        LRN_HIP(hipGetLastError());
+        return LRN_OK;
+    }
+    const dim3 grid(static_cast<unsigned>(std::max<long long>(1, (rows + kBcRows - 1) / kBcRows)),
+        static_cast<unsigned>((cols + kBcCols - 1) / kBcCols));
+    hipLaunchKernelGGL((bc_update_kernel<T, W>), grid, dim3(256), 0, s, A, lda, n, k, w, c0, c1);
+    LRN_HIP(hipGetLastError());
+    return LRN_OK;
+}
+
+// Where a block column's factorisation sends the update of the columns right of it.
+//   sx == nullptr: every inner panel's interchanges (on the block column's earlier columns
+//     too), U12 rows and update cover [k+w, KE) in one launch each, on s;
+//   sx != nullptr: the part [KB, KE) (the next block column) goes to stream sx, inner panel by
+//     inner panel: sx waits `step` (recorded on s after each panel) and runs that panel's
+//     interchanges, U12 rows and update there while s continues with the next panel. The
+//     panel's interchanges on the block column's earlier columns (their L) go to sx as well,
+//     in the same launch: sx reads those columns for the earlier panels' updates, and s
+//     never reads them again. At the end s waits `ext_done` (recorded on sx).
+struct NextColumns {
+    long long KE = 0;
+    hipStream_t sx = nullptr;
+    hipEvent_t step = nullptr;
+    hipEvent_t ext_done = nullptr;
+};
+
+// Factors block column [K, K+B) (rows K..n-1) inner panel by inner panel: each inner panel's
+// interchanges, U12 rows and A22 update are applied to the rest of the block column and to the
+// columns [K+B, nc.KE) (the next block column, or none).
 template <typename T, int W>
 int factor_block_column(T* A, long long n, long long K, long long B, long long* piv, const PanelWs<T>& ws,
-    int* rowmaps, unsigned& base, hipStream_t s, const CaluWs<float>* calu, unsigned& cbase)
+    int* rowmaps, unsigned& base, hipStream_t s, const CaluWs<float>* calu, unsigned& cbase, const NextColumns& nc)
 {
     const long long lda = n;
     const long long KB = K + B;
+    const long long KE = std::max(KB, nc.KE);
+    const bool split = nc.sx != nullptr && KE > KB;
     const int rm_stride = 1 + 4 * W;
     int slot = 0;
     for (long long k = K; k < KB; k += W, ++slot) {
@@ -751,38 +812,21 @@ int factor_block_column(T* A, long long n, long long K, long long B, long long* 
             (void)cbase;
             LRN_TRY((launch_panel<T, W>(A, lda, k, n - k, w, piv, pw, base, s)));
         }
-        // the panel's row moves on the block column's other columns and its U12 rows
-        LRN_TRY((launch_laswp_trsm<T, W>(A, lda, pw.rowmap, 2 * W, K, k, w, KB, s)));
-        if (k + w < KB) {
-            // A22 -= L21 U12 inside the block column
-            const long long rows = n - (k + w), cols = KB - (k + w);
-            const int kind = g_block_update.load();
-            if (kind == 1) {
-                LRN_TRY(launch_gemm_sub<T>(A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda,
-                    A + (k + w) * lda + (k + w), lda, rows, cols, w, s));
-                continue;
-            }
-            if (kind == 2) {
-                // matrix cores, 64 x 64 tiles, the whole K = w in one LDS stage
-                const long long tm = (rows + 63) / 64, tn = (cols + 63) / 64;
-                TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
-                if (w <= 32)
-                    hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32, true>), dim3(static_cast<unsigned>(tm * tn)), dim3(256),
-                        0, s, A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
-                        static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
-                else
-                    hipLaunchKernelGGL((gemm_sub_kernel<T, 64, 64, 32>), dim3(static_cast<unsigned>(tm * tn)), dim3(256), 0,
-                        s, A + (k + w) * lda + k, lda, A + k * lda + (k + w), lda, A + (k + w) * lda + (k + w), lda,
-                        static_cast<int>(rows), static_cast<int>(cols), w, static_cast<int>(tn));
-                LRN_HIP(hipGetLastError());
-                continue;
-            }
-            const dim3 grid(static_cast<unsigned>(std::max<long long>(1, (rows + kBcRows - 1) / kBcRows)),
-                static_cast<unsigned>((cols + kBcCols - 1) / kBcCols));
-            TraceScope ts(kTrBcUpdate, 2.0 * rows * cols * w, s);
-            hipLaunchKernelGGL((bc_update_kernel<T, W>), grid, dim3(256), 0, s, A, lda, n, k, w, k + w, KB);
-            LRN_HIP(hipGetLastError());
+        if (!split) {
+            LRN_TRY((launch_laswp_trsm<T, W>(A, lda, pw.rowmap, 2 * W, K, k, w, k + w, KE, s)));
+            LRN_TRY((launch_block_update<T, W>(A, lda, n, k, w, k + w, KE, s)));
+            continue;
         }
+        LRN_HIP(hipEventRecord(nc.step, s));
+        LRN_HIP(hipStreamWaitEvent(nc.sx, nc.step, 0));
+        LRN_TRY((launch_laswp_trsm<T, W>(A, lda, pw.rowmap, 2 * W, K, k, w, KB, KE, nc.sx)));
+        LRN_TRY((launch_block_update<T, W>(A, lda, n, k, w, KB, KE, nc.sx)));
+        LRN_TRY((launch_laswp_trsm<T, W>(A, lda, pw.rowmap, 2 * W, k, k, w, k + w, KB, s)));
+        LRN_TRY((launch_block_update<T, W>(A, lda, n, k, w, k + w, KB, s)));
+    }
+    if (split) {
+        LRN_HIP(hipEventRecord(nc.ext_done, nc.sx));
+        LRN_HIP(hipStreamWaitEvent(s, nc.ext_done, 0));
     }
     return LRN_OK;
 }
@@ -820,9 +864,10 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
     return LRN_OK;
 }
 
-// Single-stream order of the blocked factorisation: per block column, the interchanges on
-// the columns left and right of it, its U12 rows, the trailing update, then the next block
-// column's factorisation.
+// Single-stream order of the blocked factorisation. Block column i is factored together with
+// the update of block column i+1 by its inner panels; then block column i+2 and the far
+// columns [K_{i+3}, n) receive block column i's interchanges, U12 rows and update, and the
+// columns left of it its interchanges.
 template <typename T, int W>
 int getrf_blocked_plain(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd,
     const CaluWs<float>* calu)
@@ -833,32 +878,42 @@ int getrf_blocked_plain(T* A, long long n, long long* piv, const PanelWs<T>& ws,
     unsigned base = 0;
     unsigned cbase = 0;  // arrival counter of the tournament panels (cleared with the panel state)
     const int set_stride = kMaxInner * (1 + 4 * kMaxW);
-    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
+    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) % kRowmapSets) * set_stride; };
     LRN_TRY(reset_panel_state(ws, s));
-    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s, calu, cbase)));
+    NextColumns nc;
+    nc.KE = std::min(2 * NB, n);
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, s, calu, cbase, nc)));
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
+        const long long K2 = std::min(KB + NB, n);  // end of block column i+1
         const int nmaps = static_cast<int>((B + W - 1) / W);
-        LRN_TRY(launch_laswp_multi<T>(A, lda, maps(K), nmaps, rm_stride, 2 * W, 0, K, KB, n, s));
+        LRN_TRY(launch_laswp_multi<T>(A, lda, maps(K), nmaps, rm_stride, 2 * W, 0, K, K2, n, s));
         if (KB >= n) break;
-        LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, n, s)));
-        LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB,
-            n - KB, B, upd, s));
-        LRN_TRY((factor_block_column<T, W>(A, n, KB, std::min(NB, n - KB), piv, ws, maps(KB), base, s, calu, cbase)));
+        if (K2 < n) {
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, K2, n, s)));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K2, lda, A + KB * lda + K2, lda, n - KB,
+                n - K2, B, upd, s));
+        }
+        nc.KE = std::min(K2 + NB, n);
+        LRN_TRY((factor_block_column<T, W>(A, n, KB, K2 - KB, piv, ws, maps(KB), base, s, calu, cbase, nc)));
     }
     return LRN_OK;
 }
 
-// The blocked factorisation, right-looking, optionally with a look-ahead of one block column.
-// With the look-ahead, for block column K (already factored; its row maps are in ws.rowmap):
-//   gemm stream  : interchanges on columns [0, K) and on the far columns [KB+B2, n), their
+// The blocked factorisation, right-looking, optionally with a look-ahead. For block column i
+// = [K, KB) (the panel stream factors it; its inner panels' updates of block column i+1 run on
+// the narrow stream as each panel finishes, so block column i+1 is ready when block column i
+// is), with K2, K3 the ends of block columns i+1 and i+2:
+//   narrow stream: block column i+2's interchanges, U12 rows and update by block column i,
+//                  then block column i+1's inner-panel updates of block column i+2;
+//   gemm stream  : interchanges on the columns [0, K) and on the far columns [K3, n), their
 //                  U12 rows, and the far part of A22 -= L21 * U12;
-//   narrow stream: interchanges, U12 rows and the A22 update of the next block column
-//                  [KB, KB+B2) only (the critical path);
-//   panel stream : factorisation of that next block column, as soon as it is updated,
-// so the next block column's panels run while the far update of this one is still going.
-// Nothing waits on the host.
+//   panel stream : the factorisation of block column i+1 as soon as block column i is done,
+// so the panels of the next block column run while the far update of this one is still going
+// and no block column waits for a separate update of its own columns. Nothing waits on the
+// host. Results are bitwise those of getrf_blocked_plain (the same kernels on the same
+// column ranges, each entry computed independently of how the ranges are split).
 template <typename T, int W>
 int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipStream_t s, int upd = -1,
     const CaluWs<float>* calu = nullptr)
@@ -869,7 +924,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     unsigned base = 0;
     unsigned cbase = 0;  // arrival counter of the tournament panels (cleared with the panel state)
     const int set_stride = kMaxInner * (1 + 4 * kMaxW);
-    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) & 1) * set_stride; };
+    auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) % kRowmapSets) * set_stride; };
     if (g_lookahead.load() == 0 || n <= NB) return getrf_blocked_plain<T, W>(A, n, piv, ws, s, upd, calu);
     // compute units of the panel stream: a multiple of 8 (the same number on every XCD) and
     // enough to hold every cooperative workgroup of the largest (first) panel at once
@@ -894,37 +949,47 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     LRN_HIP(hipStreamWaitEvent(side->panel, side->start, 0));
     LRN_HIP(hipStreamWaitEvent(side->gemm, side->start, 0));
     LRN_TRY(reset_panel_state(ws, sn));
-    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, sn, calu, cbase)));
+    // the first block column and its update of the second: one stream (all compute units)
+    NextColumns first;
+    first.KE = std::min(2 * NB, n);
+    LRN_TRY((factor_block_column<T, W>(A, n, 0, std::min(NB, n), piv, ws, maps(0), base, sn, calu, cbase, first)));
     LRN_HIP(hipEventRecord(side->done, sn));
+    NextColumns nc;
+    nc.sx = sn;
+    nc.step = side->step;
+    nc.ext_done = side->ext_done;
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
         const int nmaps = static_cast<int>((B + W - 1) / W);
         int* const rms = maps(K);
-        const long long B2 = std::min(NB, n - KB);  // next block column (0 at the end)
-        const long long far0 = KB + B2;
-        // far columns and the left columns: GEMM stream, after block column K is factored
+        const long long K2 = std::min(KB + NB, n);  // end of block column i+1
+        const long long K3 = std::min(K2 + NB, n);  // end of block column i+2
+        // left columns and far columns: GEMM stream, after block column i is factored
         LRN_HIP(hipStreamWaitEvent(side->gemm, side->done, 0));
-        LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, far0, n, side->gemm));
-        if (far0 < n) {
-            LRN_TRY((u12_columns<T, W>(A, n, K, B, far0, n, side->gemm)));
-            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + far0, lda, A + KB * lda + far0, lda, n - KB,
-                n - far0, B, upd, side->gemm, gemm_cus));
+        LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, K3, n, side->gemm));
+        if (K3 < n) {
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, K3, n, side->gemm)));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K3, lda, A + KB * lda + K3, lda, n - KB,
+                n - K3, B, upd, side->gemm, gemm_cus));
         }
         LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
-        // next block column: narrow stream, then its factorisation on the panel stream
+        if (KB >= n) break;
+        // block column i+2 by block column i: narrow stream (its columns were in block column
+        // i-1's far update, which the narrow stream waited for at the end of the last step)
         LRN_HIP(hipStreamWaitEvent(sn, side->done, 0));
-        if (B2 > 0) {
-            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, KB, KB, KB, far0, sn));
-            LRN_TRY((u12_columns<T, W>(A, n, K, B, KB, far0, sn)));
-            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + KB, lda, A + KB * lda + KB, lda, n - KB, B2,
-                B, upd, sn));
-            LRN_HIP(hipEventRecord(side->ready, sn));
-            LRN_HIP(hipStreamWaitEvent(side->panel, side->ready, 0));
-            LRN_TRY((factor_block_column<T, W>(A, n, KB, B2, piv, ws, maps(KB), base, side->panel, calu, cbase)));
-            LRN_HIP(hipEventRecord(side->done, side->panel));
+        if (K3 > K2) {
+            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, K2, K2, K2, K3, sn));
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, K2, K3, sn)));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K2, lda, A + KB * lda + K2, lda, n - KB,
+                K3 - K2, B, upd, sn));
         }
-        // the next step's narrow work needs this step's far update (its columns)
+        // block column i+1 on the panel stream, its updates of block column i+2 on the narrow one
+        LRN_HIP(hipStreamWaitEvent(side->panel, side->done, 0));
+        nc.KE = K3;
+        LRN_TRY((factor_block_column<T, W>(A, n, KB, K2 - KB, piv, ws, maps(KB), base, side->panel, calu, cbase, nc)));
+        LRN_HIP(hipEventRecord(side->done, side->panel));
+        // the next step's narrow work (block column i+3) needs this step's far update
         LRN_HIP(hipStreamWaitEvent(sn, side->rest_done, 0));
     }
     // hand back: the caller's stream continues once all three streams are done

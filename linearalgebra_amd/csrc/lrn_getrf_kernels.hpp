@@ -276,7 +276,8 @@ constexpr int kSwapCols = 64;
 // (a row-move kernel followed by trsm_llnu_kernel would be two latency-bound launches on
 // little data).
 // Workgroups [0, na) take 64-column strips of [K, k) (moves only), workgroups [na, ..) strips
-// of [k+w, KB). All loads of a workgroup are issued together: the moved rows, and for the
+// of [rs, KB) (usually rs = k + w; the next block column's columns when the factorisation
+// updates it inner panel by inner panel). All loads of a workgroup are issued together: the moved rows, and for the
 // right strips L11 and the strip's rows k..k+w-1 as they were; A12 after the moves is then
 // assembled in LDS (a moved row's values from the gathered rows, the others as loaded), so
 // the kernel makes one memory round trip before its stores. One thread per column then solves
@@ -284,14 +285,14 @@ constexpr int kSwapCols = 64;
 // subtraction -- bitwise the result of trsm_llnu_kernel.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void laswp_trsm_kernel(T* __restrict__ A, long long lda,
-    const int* __restrict__ rowmap, int W2, long long K, long long k, int w, long long KB, int na)
+    const int* __restrict__ rowmap, int W2, long long K, long long k, int w, long long rs, long long KB, int na)
 {
     __shared__ T buf[2 * W][kSwapCols];
     __shared__ T L11[W][W + 1];
     __shared__ T X[W][kSwapCols + 1];
     const int tid = threadIdx.x;
     const bool right = static_cast<int>(blockIdx.x) >= na;
-    const long long cb = right ? k + w + static_cast<long long>(blockIdx.x - na) * kSwapCols
+    const long long cb = right ? rs + static_cast<long long>(blockIdx.x - na) * kSwapCols
                                : K + static_cast<long long>(blockIdx.x) * kSwapCols;
     const long long ce = right ? KB : k;
     const int ncols = static_cast<int>(min(static_cast<long long>(kSwapCols), ce - cb));

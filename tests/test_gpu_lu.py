@@ -226,12 +226,14 @@ def test_outer_block_widths_match_oracle(outer_nb, dev):
     assert O.rel_diff(lu_g, lu_o) <= 1e-12
 
 
-@pytest.mark.parametrize("n,dtype", [(1100, "f64"), (2100, "f64"), (2100, "f32")])
-def test_lookahead_matches_plain_order(n, dtype, dev):
+@pytest.mark.parametrize("kind", [None, 0, 1])
+@pytest.mark.parametrize("n,dtype", [(300, "f64"), (1100, "f64"), (2100, "f64"), (2100, "f32"), (2600, "f32")])
+def test_lookahead_matches_plain_order(n, dtype, kind, dev):
     """The look-ahead only reorders launches over three streams (the next block column's
-    update and panels beside the rest of the trailing update); every entry sees the same
-    operations in the same order, so factors, pivots and the solution are bitwise those of
-    the single-stream order, which matches the oracle."""
+    inner-panel updates and the block column after it beside the panels, the far trailing
+    update beside both); every entry sees the same operations in the same order, so factors,
+    pivots and the solution are bitwise those of the single-stream order, which matches the
+    oracle. kind: the block-column update kernel (None: the tuned one)."""
     from linearalgebra_amd import _lib
     lib = _lib.load()
     a = O.gaussian(n, seed=300 + n)
@@ -240,12 +242,15 @@ def test_lookahead_matches_plain_order(n, dtype, dev):
     b = O.gaussian(n, 1, seed=301 + n)[:, 0].astype(a.dtype)
     out = {}
     try:
+        if kind is not None:
+            assert lib.lrn_set_block_update(kind) == 0
         for mode in (0, 1):
             assert lib.lrn_set_lookahead(mode) == 0
             out[mode] = (_gpu_getrf(a, dev), _gpu_solve(a, b, dev, mode=1))
     finally:
         from linearalgebra_amd import autotune
         lib.lrn_set_lookahead(0)
+        lib.lrn_set_block_update(0)
         autotune.apply(lib)
     (lu0, piv0, info0, ok0), (_, _, x0) = out[0]
     (lu1, piv1, info1, ok1), (_, _, x1) = out[1]
