@@ -851,15 +851,15 @@ int u12_columns(T* A, long long n, long long K, long long B, long long c0, long 
     // one launch for the whole block column
     // narrow column ranges (the next block column): 8 columns per workgroup for parallelism
     const bool narrow = c1 - c0 <= 256;
-    const int tn = narrow ? kU12TNNarrow : kU12TN;
+    const int tn = narrow ? kU12TNNarrow : u12_tn_wide<T>();
     const long long blocks = (c1 - c0 + tn - 1) / tn;
     TraceScope ts(kTrTrsm, (static_cast<double>(B) * B + 2.0 * B * static_cast<double>(c1 - c0)) * sizeof(T), s);
     if (narrow)
         hipLaunchKernelGGL((u12_block_kernel<T, kU12TNNarrow>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A,
             lda, K, static_cast<int>(B), c0, c1);
     else
-        hipLaunchKernelGGL((u12_block_kernel<T, kU12TN>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A, lda,
-            K, static_cast<int>(B), c0, c1);
+        hipLaunchKernelGGL((u12_block_kernel<T, u12_tn_wide<T>()>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+            A, lda, K, static_cast<int>(B), c0, c1);
     LRN_HIP(hipGetLastError());
     return LRN_OK;
 }

@@ -666,10 +666,14 @@ __global__ __launch_bounds__(256) void laswp_block_kernel(T* __restrict__ A, lon
 // the 32 x 32 diagonal block. Every entry receives its updates in increasing pivot order with
 // each product rounded before the subtraction, which is exactly the row-wise forward
 // substitution of src/matrix_operation/cpu/native/lu_decomposition_native.cpp:364-380, so the
-// result does not depend on the grouping, nor on TN (columns per workgroup: 64 for wide
-// column ranges; 16 for the next block column alone, which is on the look-ahead's critical
-// path and would otherwise give only a few workgroups).
-constexpr int kU12TN = 64, kU12TNNarrow = 8;
+// result does not depend on the grouping, nor on TN (columns per workgroup). Wide column
+// ranges: 32 (FP32) or 16 (FP64), the most columns that still leave room for several
+// workgroups per compute unit (LDS: 256 x (TN + 1) entries of U); on 224 compute units, the
+// far columns [768, 16384) of n = 16384 take 114 us with TN = 32 against 207 us with 64 in
+// FP32, and 283 us with TN = 16 against 524 us with 64 in FP64. One block column's 256
+// columns alone: 8, for enough workgroups.
+template <typename T> constexpr int u12_tn_wide() { return sizeof(T) == 4 ? 32 : 16; }
+constexpr int kU12TNNarrow = 8;
 template <typename T, int TN>
 __global__ __launch_bounds__(256) void u12_block_kernel(T* __restrict__ A, long long lda, long long K, int B,
     long long c0, long long c1)
