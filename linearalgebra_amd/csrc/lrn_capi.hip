@@ -926,8 +926,11 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     const int set_stride = kMaxInner * (1 + 4 * kMaxW);
     auto maps = [&](long long K0) { return ws.rowmap + ((K0 / NB) % kRowmapSets) * set_stride; };
     if (g_lookahead.load() == 0 || n <= NB) return getrf_blocked_plain<T, W>(A, n, piv, ws, s, upd, calu);
-    // compute units of the panel stream: a multiple of 8 (the same number on every XCD) and
-    // enough to hold every cooperative workgroup of the largest (first) panel at once
+    // compute units of the panel stream: a multiple of 8 (the same number on every XCD),
+    // enough to hold every cooperative workgroup of the largest (first) panel at once, and one
+    // more per XCD: the narrow stream's work runs beside the panels and may hold slots on
+    // these units, so an exact fit can leave a panel unable to become resident (at n = 16384,
+    // 64 panel workgroups on 16 units timed out their bounded wait)
     const int la_mode = g_lookahead.load();
     int np = la_mode == 1 ? 32 : la_mode;
     int dev = 0;
@@ -936,7 +939,7 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         int need = 0;
         unsigned base_q = 0;
         LRN_TRY((launch_panel<T, W>(A, lda, 0, n, W, piv, ws, base_q, s, &need)));
-        np = std::max(np, (need + kXcds - 1) / kXcds * kXcds);
+        np = std::max(np, (need + kXcds - 1) / kXcds * kXcds + kXcds);
     }
     if (np > device_props(dev).cus / 2) return getrf_blocked_plain<T, W>(A, n, piv, ws, s, upd, calu);
     AsideStream* side = nullptr;

@@ -226,6 +226,51 @@ def test_outer_block_widths_match_oracle(outer_nb, dev):
     assert O.rel_diff(lu_g, lu_o) <= 1e-12
 
 
+@pytest.mark.parametrize("mode", [16, 32, 64])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_lookahead_panel_units_at_n16384(mode, dtype, dev):
+    """Every panel-stream size the autotuner tries completes the n = 16384 factorisation: the
+    panel stream keeps one compute unit per XCD beyond what the largest panel's cooperative
+    workgroups need, since the narrow stream's kernels run beside the panels on those units
+    (an exact fit, 64 workgroups on 16 units, timed out its bounded wait). The factors are
+    checked through P A = L U on a sample of rows."""
+    import torch
+    import linearalgebra_amd as la
+    from linearalgebra_amd import _lib, autotune
+    lib = _lib.load()
+    n = 16384
+    dt = torch.float32 if dtype == "f32" else torch.float64
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    a = torch.randn(n, n, dtype=dt, device=dev, generator=g)
+    w = a.clone()
+    piv = torch.empty(n, dtype=torch.int64, device=dev)
+    ws = torch.empty(la.lu_decomposition.workspace_size(n, dt), dtype=torch.uint8, device=dev)
+    try:
+        assert lib.lrn_set_lookahead(mode) == 0
+        la.lu_decomposition.apply(w, n, piv, ws)
+        torch.cuda.synchronize()
+    finally:
+        lib.lrn_set_lookahead(0)
+        autotune.apply(lib)
+    p = piv.cpu().numpy()
+    assert np.all(p >= np.arange(1, n + 1)) and np.all(p <= n)
+    perm = np.arange(n)
+    for i in range(n):
+        j = int(p[i]) - 1
+        perm[i], perm[j] = perm[j], perm[i]
+    rows = torch.arange(0, n, 97, device=dev)
+    lu = w.double()
+    cols = torch.arange(n, device=dev)
+    L = lu[rows] * (cols[None, :] < rows[:, None])  # strictly lower part of the sampled rows
+    L[torch.arange(rows.numel(), device=dev), rows] = 1.0
+    rec = L @ torch.triu(lu)
+    ref = a.double()[torch.from_numpy(perm).to(dev)[rows]]
+    rel = ((rec - ref).abs().max() / a.abs().max()).item()
+    print(f"lookahead={mode} {dtype} n={n}: sampled |PA - LU| / max|A| = {rel:.2e}", flush=True)
+    assert rel <= (1e-3 if dtype == "f32" else 1e-10)
+
+
 @pytest.mark.parametrize("kind", [None, 0, 1])
 @pytest.mark.parametrize("n,dtype", [(300, "f64"), (1100, "f64"), (2100, "f64"), (2100, "f32"), (2600, "f32")])
 def test_lookahead_matches_plain_order(n, dtype, kind, dev):
