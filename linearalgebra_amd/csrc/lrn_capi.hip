@@ -657,6 +657,7 @@ struct AsideStream {
     hipEvent_t ready = nullptr;      // recorded on the narrow stream: next block column updated
     hipEvent_t done = nullptr;       // recorded on the panel stream: next block column factored
     hipEvent_t rest_done = nullptr;  // recorded on the GEMM stream: rest of the trailing update done
+    hipEvent_t next_ready = nullptr; // recorded on the GEMM stream: block column i+2 updated by block column i
     hipEvent_t step = nullptr;       // recorded on the panel stream after each inner panel
     hipEvent_t ext_done = nullptr;   // recorded on the narrow stream: next block column's inner updates done
     std::mutex mu;
@@ -685,6 +686,7 @@ void release_aside_streams()
             (void)hipEventDestroy(a->done);
             (void)hipEventDestroy(a->rest_done);
             (void)hipEventDestroy(a->step);
+            (void)hipEventDestroy(a->next_ready);
             (void)hipEventDestroy(a->ext_done);
             (void)hipStreamDestroy(a->narrow);
             (void)hipStreamDestroy(a->panel);
@@ -721,6 +723,7 @@ int aside_stream(int np, AsideStream** out)
     LRN_HIP(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
     LRN_HIP(hipEventCreateWithFlags(&a->rest_done, hipEventDisableTiming));
     LRN_HIP(hipEventCreateWithFlags(&a->step, hipEventDisableTiming));
+    LRN_HIP(hipEventCreateWithFlags(&a->next_ready, hipEventDisableTiming));
     LRN_HIP(hipEventCreateWithFlags(&a->ext_done, hipEventDisableTiming));
     *out = a.get();
     per_dev[dev].push_back(std::move(a));
@@ -905,10 +908,11 @@ int getrf_blocked_plain(T* A, long long n, long long* piv, const PanelWs<T>& ws,
 // = [K, KB) (the panel stream factors it; its inner panels' updates of block column i+1 run on
 // the narrow stream as each panel finishes, so block column i+1 is ready when block column i
 // is), with K2, K3 the ends of block columns i+1 and i+2:
-//   narrow stream: block column i+2's interchanges, U12 rows and update by block column i,
-//                  then block column i+1's inner-panel updates of block column i+2;
-//   gemm stream  : interchanges on the columns [0, K) and on the far columns [K3, n), their
-//                  U12 rows, and the far part of A22 -= L21 * U12;
+//   gemm stream  : block column i+2's interchanges, U12 rows and update by block column i
+//                  (event next_ready), then the interchanges on the columns [0, K) and on the
+//                  far columns [K3, n), their U12 rows, and the far part of A22 -= L21 * U12;
+//   narrow stream: after next_ready, block column i+1's inner-panel updates of block column
+//                  i+2 (and their interchanges on block column i+1's earlier columns);
 //   panel stream : the factorisation of block column i+1 as soon as block column i is done,
 // so the panels of the next block column run while the far update of this one is still going
 // and no block column waits for a separate update of its own columns. Nothing waits on the
@@ -961,6 +965,9 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
     nc.sx = sn;
     nc.step = side->step;
     nc.ext_done = side->ext_done;
+    // the step ending at column KB0 updates the block column after next on the narrow stream
+    // (FP64 over the first 30 % of the columns, where the far update is the critical path)
+    auto next_on_narrow = [&](long long KB0) { return std::is_same<T, double>::value && 10 * (n - KB0) > 7 * n; };
     for (long long K = 0; K < n; K += NB) {
         const long long B = std::min(NB, n - K);
         const long long KB = K + B;
@@ -968,8 +975,21 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         int* const rms = maps(K);
         const long long K2 = std::min(KB + NB, n);  // end of block column i+1
         const long long K3 = std::min(K2 + NB, n);  // end of block column i+2
-        // left columns and far columns: GEMM stream, after block column i is factored
+        // block column i+2's interchanges, U12 rows and update by block column i: on the GEMM
+        // stream ahead of the far columns (event next_ready; on the GEMM stream's own compute
+        // units they do not slow the panels, as they do from the narrow stream, whose kernels
+        // share the panel's units while the far GEMM fills the others), except while the FP64
+        // far update outlasts the panels, where they would lengthen the critical GEMM stream
+        const bool next_on_gemm = !next_on_narrow(KB);
         LRN_HIP(hipStreamWaitEvent(side->gemm, side->done, 0));
+        if (next_on_gemm && K3 > K2) {
+            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, K2, K2, K2, K3, side->gemm));
+            LRN_TRY((u12_columns<T, W>(A, n, K, B, K2, K3, side->gemm)));
+            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K2, lda, A + KB * lda + K2, lda, n - KB,
+                K3 - K2, B, upd, side->gemm, gemm_cus));
+        }
+        if (next_on_gemm) LRN_HIP(hipEventRecord(side->next_ready, side->gemm));
+        // then the left columns' interchanges and the far columns
         LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, 0, K, K3, n, side->gemm));
         if (K3 < n) {
             LRN_TRY((u12_columns<T, W>(A, n, K, B, K3, n, side->gemm)));
@@ -978,22 +998,27 @@ int getrf_blocked(T* A, long long n, long long* piv, const PanelWs<T>& ws, hipSt
         }
         LRN_HIP(hipEventRecord(side->rest_done, side->gemm));
         if (KB >= n) break;
-        // block column i+2 by block column i: narrow stream (its columns were in block column
-        // i-1's far update, which the narrow stream waited for at the end of the last step)
-        LRN_HIP(hipStreamWaitEvent(sn, side->done, 0));
-        if (K3 > K2) {
-            LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, K2, K2, K2, K3, sn));
-            LRN_TRY((u12_columns<T, W>(A, n, K, B, K2, K3, sn)));
-            LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K2, lda, A + KB * lda + K2, lda, n - KB,
-                K3 - K2, B, upd, sn));
+        if (next_on_gemm) {
+            LRN_HIP(hipStreamWaitEvent(sn, side->next_ready, 0));
+        } else {
+            // on the narrow stream (it waited for block column i-1's far update, which covered
+            // these columns, at the end of the last step)
+            LRN_HIP(hipStreamWaitEvent(sn, side->done, 0));
+            if (K3 > K2) {
+                LRN_TRY(launch_laswp_multi<T>(A, lda, rms, nmaps, rm_stride, 2 * W, K2, K2, K2, K3, sn));
+                LRN_TRY((u12_columns<T, W>(A, n, K, B, K2, K3, sn)));
+                LRN_TRY(launch_trailing<T>(A + KB * lda + K, lda, A + K * lda + K2, lda, A + KB * lda + K2, lda,
+                    n - KB, K3 - K2, B, upd, sn));
+            }
         }
         // block column i+1 on the panel stream, its updates of block column i+2 on the narrow one
         LRN_HIP(hipStreamWaitEvent(side->panel, side->done, 0));
         nc.KE = K3;
         LRN_TRY((factor_block_column<T, W>(A, n, KB, K2 - KB, piv, ws, maps(KB), base, side->panel, calu, cbase, nc)));
         LRN_HIP(hipEventRecord(side->done, side->panel));
-        // the next step's narrow work (block column i+3) needs this step's far update
-        LRN_HIP(hipStreamWaitEvent(sn, side->rest_done, 0));
+        // block column i+3's update by block column i+1 needs this step's far update: on the
+        // GEMM stream by order, on the narrow stream by this wait
+        if (next_on_narrow(K2)) LRN_HIP(hipStreamWaitEvent(sn, side->rest_done, 0));
     }
     // hand back: the caller's stream continues once all three streams are done
     LRN_HIP(hipEventRecord(side->ready, sn));
