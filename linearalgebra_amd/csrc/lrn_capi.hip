@@ -641,10 +641,12 @@ template <typename T> int reset_panel_state(const PanelWs<T>& ws, hipStream_t s)
 // goes to these streams (the caller's stream only hands over and takes back through events):
 // the legacy NULL stream, which callers commonly pass, waits for all work on blocking streams
 // before each of its launches, so work left on it would serialise the three streams.
-//   narrow: the first block column and each next block column's interchanges, U12 rows and
-//           update (the critical path); non-blocking, all compute units;
-//   panel : the block-column factorisations, restricted to `np` compute units so the
-//           cooperative panel's workgroups always find room;
+//   narrow: the first block column and each inner panel's update of the columns beyond its
+//           own block column, restricted to the GEMM stream's compute units (from the panel's
+//           units its kernels slowed the panels beside them: mixed n = 16384 solve 114.9 ms
+//           against 114.4 ms on the same box);
+//   panel : the inner panels and their own block column's updates, restricted to `np`
+//           compute units so the cooperative panel's workgroups always find room;
 //   gemm  : the rest of each trailing update, restricted to the other compute units.
 // Default priority. `mu` keeps one factorisation at a time enqueuing on a set, so the
 // events below always refer to the same call.
@@ -715,7 +717,7 @@ int aside_stream(int np, AsideStream** out)
     const int words = (cus + 31) / 32;
     std::vector<uint32_t> mp(words, 0u), mg(words, 0u);
     for (int c = 0; c < cus; ++c) (c < np ? mp : mg)[c / 32] |= 1u << (c % 32);
-    LRN_HIP(hipStreamCreateWithFlags(&a->narrow, hipStreamNonBlocking));
+    LRN_HIP(hipExtStreamCreateWithCUMask(&a->narrow, static_cast<uint32_t>(words), mg.data()));
     LRN_HIP(hipExtStreamCreateWithCUMask(&a->panel, static_cast<uint32_t>(words), mp.data()));
     LRN_HIP(hipExtStreamCreateWithCUMask(&a->gemm, static_cast<uint32_t>(words), mg.data()));
     LRN_HIP(hipEventCreateWithFlags(&a->start, hipEventDisableTiming));
