@@ -2118,8 +2118,13 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
         }
     } guard{e0, e1, e2};
     LRN_HIP(hipEventRecord(e0, s));
-    const long long nn = n * n;
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(nn)), dim3(256), 0, s, A, L.a32, nn);
+    // A in FP32 and the row sums of |A| (for ||A||_inf) in one pass over A
+    if (n % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(L.a32) % 8 == 0)
+        hipLaunchKernelGGL(convert_rowsum_kernel<true>, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n,
+            L.a32, L.rowsum);
+    else
+        hipLaunchKernelGGL(convert_rowsum_kernel<false>, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A,
+            n, L.a32, L.rowsum);
     LRN_HIP(hipGetLastError());
     LRN_TRY((getrf_blocked<float, 32>(L.a32, n, L.piv, L.f.ws, s, upd, g_mixed_panel.load() ? &L.calu : nullptr)));
     LRN_TRY(getrs_perm<double>(n, L.piv, L.g, s));
@@ -2134,9 +2139,6 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     st.info = info;
 
     MixedSolver S{A, n, b, x, L, s};
-    hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(256), 0, s, A, n, nullptr,
-        nullptr, L.rowsum, 0.0, 0.0, 1);
-    LRN_HIP(hipGetLastError());
     LRN_TRY(S.reduce(L.rowsum, nullptr, 0, &S.anorm));
     LRN_TRY(S.reduce(b, nullptr, 0, &S.bnorm));
 
@@ -2206,7 +2208,7 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     }
     int fb_info = 0, fb_err = 0;
     if (!done && fallback) {
-        LRN_HIP(hipMemcpyAsync(L.afb, A, sizeof(double) * static_cast<size_t>(nn), hipMemcpyDeviceToDevice, s));
+        LRN_HIP(hipMemcpyAsync(L.afb, A, sizeof(double) * static_cast<size_t>(n * n), hipMemcpyDeviceToDevice, s));
         LRN_TRY(getrf_dispatch<double>(L.afb, n, L.pivfb, L.ffb.ws, s));
         LRN_TRY(S.copy(x, b));
         LRN_TRY(launch_gate<double>(L.ffb.ws.info, L.ffb.ws.err, L.gfb.gate, x, n, 1, 1, s));

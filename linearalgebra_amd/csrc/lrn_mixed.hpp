@@ -48,6 +48,44 @@ __global__ __launch_bounds__(256) void gemv_rows_kernel(const double* __restrict
     }
 }
 
+// The FP32 copy of A (n x n, row-major, as convert_kernel) fused with the row sums of |A| for
+// ||A||_inf, so A is read once for both: one wave per row. kVec (even n, 16-byte aligned A):
+// 16-byte loads and 8-byte stores, lane l taking the column pairs 2 l + 128 k; otherwise one
+// element per lane and step. Each row's sum has a fixed order, so the norm is reproducible.
+template <bool kVec>
+__global__ __launch_bounds__(256) void convert_rowsum_kernel(const double* __restrict__ A, long long n,
+    float* __restrict__ out, double* __restrict__ rowsum)
+{
+    const int lane = threadIdx.x & 63;
+    const long long row = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* a = A + row * n;
+    float* o = out + row * n;
+    double s = 0.0;
+    if (kVec) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        typedef float f2 __attribute__((ext_vector_type(2)));
+#pragma unroll 4
+        for (long long c = 2 * lane; c < n; c += 128) {
+            const d2 v = *reinterpret_cast<const d2*>(a + c);
+            f2 w;
+            w.x = static_cast<float>(v.x);
+            w.y = static_cast<float>(v.y);
+            *reinterpret_cast<f2*>(o + c) = w;
+            s += fabs(v.x) + fabs(v.y);
+        }
+    } else {
+#pragma unroll 4
+        for (long long c = lane; c < n; c += 64) {
+            const double v = a[c];
+            o[c] = static_cast<float>(v);
+            s += fabs(v);
+        }
+    }
+    s = wave_sum_f64(s);
+    if (lane == 0) rowsum[row] = s;
+}
+
 // The same product (mode 0) with 16-byte loads, four in flight per lane and four partial sums
 // per lane (even n, 16-byte aligned A and x): 0.35 ms against 0.41 ms at n = 16384 (6.1 TB/s).
 typedef double gemv_d2 __attribute__((ext_vector_type(2)));
