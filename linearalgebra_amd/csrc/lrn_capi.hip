@@ -1934,6 +1934,29 @@ unsigned grid_for(long long n, int threads = 256)
     return static_cast<unsigned>(std::max<long long>(1, std::min<long long>((n + threads - 1) / threads, 1 << 20)));
 }
 
+// Page-locked host words for the refinement's two norms and an event recorded after their copy,
+// per thread and device, created on first use and kept until the process ends: the copy is
+// then asynchronous, and the host waits for it alone rather than for the whole stream.
+int norms_buffer(double** words, hipEvent_t* copied)
+{
+    struct Slot {
+        double* p = nullptr;
+        hipEvent_t ev = nullptr;
+    };
+    static thread_local std::vector<Slot> slots;
+    int dev = 0;
+    LRN_HIP(hipGetDevice(&dev));
+    if (dev >= static_cast<int>(slots.size())) slots.resize(dev + 1);
+    Slot& sl = slots[dev];
+    if (sl.p == nullptr) {
+        LRN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.p), 2 * sizeof(double), hipHostMallocDefault));
+        LRN_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    }
+    *words = sl.p;
+    *copied = sl.ev;
+    return LRN_OK;
+}
+
 struct MixedSolver {
     const double* A;
     long long n;
@@ -1977,16 +2000,22 @@ struct MixedSolver {
         return getrs_subst<double, float>(L.a32, n, out, 1, 1, L.g, nullptr, s, true);
     }
     // r = b - A x and the backward error (both norms fetched with one synchronisation)
-    int residual(double* berr)
+    // next: also enqueue the next correction M^-1 r into L.d after the norms' copy, so the GPU
+    // computes it while the host waits for the norms (page-locked words, an event after the copy)
+    int residual(double* berr, bool next = false)
     {
         LRN_TRY(gemv(x, b, L.r, -1.0, 1.0));
         hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, L.r, nullptr, n, 0, L.scal);
         LRN_HIP(hipGetLastError());
         hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, x, nullptr, n, 0, L.scal + 1);
         LRN_HIP(hipGetLastError());
-        double norms[2] = {0.0, 0.0};
-        LRN_HIP(hipMemcpyAsync(norms, L.scal, sizeof(norms), hipMemcpyDeviceToHost, s));
-        LRN_HIP(hipStreamSynchronize(s));
+        double* norms = nullptr;
+        hipEvent_t copied = nullptr;
+        LRN_TRY(norms_buffer(&norms, &copied));
+        LRN_HIP(hipMemcpyAsync(norms, L.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        LRN_HIP(hipEventRecord(copied, s));
+        if (next) LRN_TRY(precond(L.r, L.d));
+        LRN_HIP(hipEventSynchronize(copied));
         const double rn = norms[0], xn = norms[1];
         const double den = anorm * xn + bnorm;
         *berr = den > 0.0 ? rn / den : rn;
@@ -2149,11 +2178,21 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
         LRN_TRY(S.axpby(x, nullptr, 0.0, 0.0));
         LRN_TRY(S.copy(L.r, b));
         double best = 1.0 / 0.0;
+        bool have_next = false;                            // L.d already holds the next correction
+        double last = 1.0 / 0.0, before = 1.0 / 0.0;  // the last two backward errors
         for (int it = 0; it < max_iter; ++it) {
-            LRN_TRY(S.precond(L.r, L.d));
+            if (!have_next) LRN_TRY(S.precond(L.r, L.d));
             LRN_TRY(S.axpby(x, L.d, 1.0, 1.0));
             ++st.ir_iterations;
-            LRN_TRY(S.residual(&berr));
+            // the next correction goes beside the host's wait for the backward error when the
+            // last two steps' rate predicts that this one stays above the tolerance (it is
+            // computed from the same residual either way, so the iterates do not change; a
+            // wasted one costs a pair of substitutions)
+            const bool next = it >= 2 && g_refinement.load() != 1 && last * (last / before) > tol;
+            LRN_TRY(S.residual(&berr, next));
+            have_next = next;
+            before = last;
+            last = berr;
             if (mixed_verbose()) std::fprintf(stderr, "lrn mixed n=%lld lu-ir %d berr %.3e\n", n, it, berr);
             const double prev_best = best;
             if (berr < best) {
