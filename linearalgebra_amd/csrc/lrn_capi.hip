@@ -2160,16 +2160,24 @@ int gesv_mixed_impl(const double* A, long long n, const double* b, double* x, do
     LRN_TRY(refine_inverses(L.a32, n, L.g, s));
     LRN_TRY(launch_fill(reinterpret_cast<unsigned*>(L.g.err), 0u, 64, s));
     LRN_HIP(hipEventRecord(e1, s));
+    // ||A||_inf (from the conversion's row sums) and ||b||, fetched with info and err in one
+    // synchronisation
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, L.rowsum, nullptr, n, 0, L.scal + 2);
+    LRN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, b, nullptr, n, 0, L.scal + 3);
+    LRN_HIP(hipGetLastError());
     int info = 0, err = 0;
+    double norms[2] = {0.0, 0.0};
     LRN_HIP(hipMemcpyAsync(&info, L.f.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
     LRN_HIP(hipMemcpyAsync(&err, L.f.ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    LRN_HIP(hipMemcpyAsync(norms, L.scal + 2, sizeof(norms), hipMemcpyDeviceToHost, s));
     LRN_HIP(hipStreamSynchronize(s));
     if (err != 0) return LRN_E_TIMEOUT;
     st.info = info;
 
     MixedSolver S{A, n, b, x, L, s};
-    LRN_TRY(S.reduce(L.rowsum, nullptr, 0, &S.anorm));
-    LRN_TRY(S.reduce(b, nullptr, 0, &S.bnorm));
+    S.anorm = norms[0];
+    S.bnorm = norms[1];
 
     double berr = 1.0 / 0.0;
     bool done = false;
